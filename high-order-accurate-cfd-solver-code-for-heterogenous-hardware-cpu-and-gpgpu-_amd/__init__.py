@@ -1,0 +1,372 @@
+"""cfdx — MI355X-native per-stage RHS + RK3 update of the high-order curvilinear
+CFD solver, behind the reference's kernel interface (U_Q / Viscous / Weno).
+
+Host-side mirror of the reference's Solver.cpp (stage sequence and argument
+layout) over the C ABI of libcfdx.so (include/cfdx.h). PyTorch is used only
+for device memory and streams; every computation runs in the HIP kernels of
+libcfdx.so, and importing this package fails loudly when that library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfdx.so")
+
+# ---------------------------------------------------------------------------
+# packed layouts (functions.h:49-112)
+# ---------------------------------------------------------------------------
+FLOW_FIELDS = ["u", "v", "w", "p", "t", "rho", "mu", "a", "e"]
+FLOW_DT = np.dtype([(f, "<f8") for f in FLOW_FIELDS])                       # FLOW_VARIABLES, 72 B
+GRID_DT = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8")] +
+                   [(f"N{i}", "<i4") for i in range(6)] + [("ID", "<i4"), ("loc", "<i4")])  # GRID, 56 B
+VEC5_N, ROE_N, METRIC_N, DETER_N = 5, 36, 63, 6                            # doubles per record
+assert FLOW_DT.itemsize == 72 and GRID_DT.itemsize == 56
+
+FP64, MIXED = 0, 1
+E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_NONFINITE, E_CAPACITY = -1, -2, -3, -4, -5, -6
+
+
+class Params(C.Structure):
+    """cfdx_params: the constants the reference hard-codes, plus precision / viscous switches."""
+    _fields_ = [("mach", C.c_double), ("reyl", C.c_double), ("pr", C.c_double), ("free_t", C.c_double),
+                ("c_mts", C.c_double), ("c_t", C.c_double), ("eps", C.c_double),
+                ("precision", C.c_int32), ("viscous", C.c_int32)]
+
+
+class BlockDesc(C.Structure):
+    _fields_ = [("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32), ("periodic", C.c_int32),
+                ("precision", C.c_int32)]
+
+
+class CfdxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cfdx error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# library
+# ---------------------------------------------------------------------------
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with build.py (hipcc, gfx950) before importing cfdx")
+
+_lib = C.CDLL(LIB_PATH)
+_vp, _i64, _i32, _d = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "cfdx_last_error": (C.c_char_p, []),
+    "cfdx_params_default": (None, [C.POINTER(Params)]),
+    "cfdx_ctx_create": (C.c_int, [C.c_int, _vp, C.POINTER(_vp)]),
+    "cfdx_ctx_destroy": (C.c_int, [_vp]),
+    "cfdx_ctx_set_stream": (C.c_int, [_vp, _vp]),
+    "cfdx_ctx_stream": (_vp, [_vp]),
+    "cfdx_sync": (C.c_int, [_vp]),
+    "cfdx_uq": (C.c_int, [_vp] * 7 + [_i64]),
+    "cfdx_viscous": (C.c_int, [_vp] * 12 + [_i64, C.POINTER(Params)]),
+    "cfdx_weno": (C.c_int, [_vp, C.c_int, _d, _d] + [_vp] * 16 + [_i64, C.POINTER(Params)]),
+    "cfdx_block_bytes": (_i64, [C.POINTER(BlockDesc)]),
+    "cfdx_block_create": (C.c_int, [_vp, C.POINTER(BlockDesc), C.POINTER(_vp)]),
+    "cfdx_block_destroy": (C.c_int, [_vp]),
+    "cfdx_block_set_geometry": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "cfdx_block_set_flow": (C.c_int, [_vp, _vp]),
+    "cfdx_block_get_flow": (C.c_int, [_vp, _vp]),
+    "cfdx_block_step": (C.c_int, [_vp, _d, _d, C.c_int, C.POINTER(Params)]),
+    "cfdx_block_rhs": (C.c_int, [_vp, C.POINTER(Params), _vp]),
+    "cfdx_block_set_timing": (C.c_int, [_vp, C.c_int]),
+    "cfdx_block_timing": (C.c_int, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(C.c_int)]),
+    "cfdx_geom_periodic_table": (C.c_int, [_i32, _i32, _i32, _vp, _vp]),
+    "cfdx_geom_coords_box": (C.c_int, [_i32, _i32, _i32, _vp, _vp]),
+    "cfdx_geom_coords_sinusoid": (C.c_int, [_i32, _i32, _i32, _vp, _d, _vp, _vp]),
+    "cfdx_geom_jacobian_sinusoid": (C.c_int, [_i32, _i32, _i32, _vp, _d, _vp, _vp]),
+    "cfdx_geom_jacobian_compact_periodic": (C.c_int, [_i32, _i32, _i32, _vp, _vp, _vp]),
+    "cfdx_geom_node_metrics": (C.c_int, [_i64, _vp, _vp, _vp]),
+    "cfdx_geom_half_metrics": (C.c_int, [_vp, _i64, _vp, _vp, _vp]),
+    "cfdx_geom_init_freestream": (C.c_int, [_i64, _d, _vp]),
+    "cfdx_geom_init_vortex": (C.c_int, [_i64, _vp, _d, _d, _d, _d, _vp]),
+    "cfdx_geom_init_tgv": (C.c_int, [_i64, _vp, _d, _vp]),
+}
+
+for _name, (_res, _args) in EXPORTS.items():
+    _f = getattr(_lib, _name)   # AttributeError here means the library lacks a declared symbol
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise CfdxError(rc, _lib.cfdx_last_error().decode())
+
+
+def _ptr(a) -> C.c_void_p:
+    """Address of a numpy array or a torch tensor (host or device)."""
+    if a is None:
+        return C.c_void_p(0)
+    if isinstance(a, np.ndarray):
+        return C.c_void_p(a.ctypes.data)
+    return C.c_void_p(a.data_ptr())
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    _lib.cfdx_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def euler_params(**kw) -> Params:
+    """Inviscid form: infinite Re, no SGS model, viscous kernels skipped."""
+    return default_params(reyl=math.inf, c_mts=0.0, viscous=0, **kw)
+
+
+# ---------------------------------------------------------------------------
+# host geometry -> reference-numbered case (node 0 = dummy)
+# ---------------------------------------------------------------------------
+def periodic_case(ni: int, nj: int, nk: int, grid: str = "box", length=(10.0, 10.0, 10.0), amp: float = 0.0,
+                  phases=(0.0, 0.0, 0.0), init: str = "freestream", mach: float = 0.5, beta: float = 5.0,
+                  center=(5.0, 5.0), metrics: str = "analytic", seed: int = 20261202, noise: float = 0.0) -> dict:
+    """Builds Grid_P, det, metric (node + half point), deter and FLOW for a periodic block.
+
+    grid: 'box' (Cartesian) or 'sinusoid' (seeded curvilinear map, amplitude amp).
+    init: 'freestream', 'vortex' (isentropic, strength beta) or 'tgv' (Taylor-Green).
+    metrics: 'analytic' (exact Jacobian of the map) or 'compact' (6th-order compact scheme).
+    noise: optional seeded smooth perturbation amplitude added to rho, u, v, w, p (tests).
+    """
+    n = ni * nj * nk
+    L = np.ascontiguousarray(length, dtype=np.float64)
+    ph = np.ascontiguousarray(phases, dtype=np.float64)
+    xyz = np.zeros((n, 3))
+    if grid == "box":
+        _check(_lib.cfdx_geom_coords_box(ni, nj, nk, _ptr(L), _ptr(xyz)))
+        a = 0.0
+    elif grid == "sinusoid":
+        _check(_lib.cfdx_geom_coords_sinusoid(ni, nj, nk, _ptr(L), amp, _ptr(ph), _ptr(xyz)))
+        a = amp
+    else:
+        raise ValueError(grid)
+    jac = np.zeros((n, 9))
+    if metrics == "analytic":
+        _check(_lib.cfdx_geom_jacobian_sinusoid(ni, nj, nk, _ptr(L), a, _ptr(ph), _ptr(jac)))
+    elif metrics == "compact":
+        _check(_lib.cfdx_geom_jacobian_compact_periodic(ni, nj, nk, _ptr(L), _ptr(xyz), _ptr(jac)))
+    else:
+        raise ValueError(metrics)
+    det = np.ones(n + 1)
+    metric = np.zeros((n + 1, METRIC_N))
+    metric[0, [0, 4, 8]] = 1.0  # dummy node: identity metrics
+    metric[0, 9:] = np.tile([1.0, 0, 0, 0, 1.0, 0, 0, 0, 1.0], 6)
+    d_node = np.zeros(n)
+    m_node = np.zeros((n, METRIC_N))
+    _check(_lib.cfdx_geom_node_metrics(n, _ptr(jac), _ptr(d_node), _ptr(m_node)))
+    det[1:] = d_node
+    metric[1:, :9] = m_node[:, :9]
+    grid_p = np.zeros(n + 1, dtype=GRID_DT)
+    _check(_lib.cfdx_geom_periodic_table(ni, nj, nk, _ptr(xyz), _ptr(grid_p)))
+    deter = np.ones((n + 1, DETER_N))
+    _check(_lib.cfdx_geom_half_metrics(_ptr(grid_p), n, _ptr(metric), _ptr(det), _ptr(deter)))
+    flow = np.zeros(n + 1, dtype=FLOW_DT)
+    if init == "freestream":
+        _check(_lib.cfdx_geom_init_freestream(n, mach, _ptr(flow)))
+    elif init == "vortex":
+        _check(_lib.cfdx_geom_init_vortex(n, _ptr(xyz), mach, beta, center[0], center[1], _ptr(flow)))
+    elif init == "tgv":
+        _check(_lib.cfdx_geom_init_tgv(n, _ptr(xyz), mach, _ptr(flow)))
+    else:
+        raise ValueError(init)
+    if noise:
+        rng = np.random.default_rng(seed)
+        x, y, z = (xyz[:, 0] * 2 * np.pi / L[0], xyz[:, 1] * 2 * np.pi / L[1], xyz[:, 2] * 2 * np.pi / L[2])
+        for f, scale in (("rho", 1.0), ("u", 1.0), ("v", 1.0), ("w", 1.0), ("p", 1.0)):
+            kx, ky, kz = rng.integers(1, 3, size=3)
+            phs = rng.uniform(0, 2 * np.pi, size=3)
+            base = np.abs(flow[f][1:]).mean() + 1.0 if f in ("rho", "p") else 1.0
+            flow[f][1:] += noise * base * scale * (np.sin(kx * x + phs[0]) * np.cos(ky * y + phs[1]) * np.sin(kz * z + phs[2]))
+        fl = flow[1:]
+        fl["t"] = 1.4 * mach * mach * fl["p"] / fl["rho"]
+        fl["e"] = fl["p"] / (0.4 * fl["rho"])
+        fl["a"] = np.sqrt(1.4 * fl["p"] / fl["rho"])
+    return dict(ni=ni, nj=nj, nk=nk, n=n, xyz=xyz, jac=jac, Grid_P=grid_p, det=det, metric=metric, deter=deter,
+                FLOW=flow)
+
+
+def cfl_dt(case: dict, cfl: float) -> float:
+    """dt = cfl / max_nodes sum_d (|u . grad xi_d| + c |grad xi_d|) on the initial state (computed once)."""
+    fl = case["FLOW"][1:]
+    m = case["metric"][1:, :9].reshape(-1, 3, 3)
+    vel = np.stack([fl["u"], fl["v"], fl["w"]], axis=1)
+    c = np.sqrt(1.4 * fl["p"] / fl["rho"])
+    s = np.zeros(len(fl))
+    for d in range(3):
+        s += np.abs(np.einsum("ni,ni->n", vel, m[:, d])) + c * np.linalg.norm(m[:, d], axis=1)
+    return float(cfl / s.max())
+
+
+# ---------------------------------------------------------------------------
+# context + drop-in kernels on device buffers (torch tensors)
+# ---------------------------------------------------------------------------
+class Ctx:
+    """A device + stream. stream=None uses torch's current stream when torch is importable."""
+
+    def __init__(self, device: int = 0, stream=None):
+        h = _vp()
+        if stream is None:
+            try:
+                import torch
+                stream = torch.cuda.current_stream(device).cuda_stream
+            except Exception:
+                stream = None
+        _check(_lib.cfdx_ctx_create(device, _vp(stream) if stream else None, C.byref(h)))
+        self.h = h
+
+    def sync(self) -> None:
+        _check(_lib.cfdx_sync(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            _check(_lib.cfdx_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference argument order (Solver.cpp:277-314)
+    def uq(self, U_C, Q_C, Grid_P, det, FLOW, ROE, n_launch: int) -> None:
+        _check(_lib.cfdx_uq(self.h, _ptr(U_C), _ptr(Q_C), _ptr(Grid_P), _ptr(det), _ptr(FLOW), _ptr(ROE), n_launch))
+
+    def viscous(self, U_C, F, E, G, Fv, Ev, Gv, Grid_P, metric, det, FLOW, n_launch: int, params: Params) -> None:
+        _check(_lib.cfdx_viscous(self.h, _ptr(U_C), _ptr(F), _ptr(E), _ptr(G), _ptr(Fv), _ptr(Ev), _ptr(Gv),
+                                 _ptr(Grid_P), _ptr(metric), _ptr(det), _ptr(FLOW), n_launch, C.byref(params)))
+
+    def weno(self, j, mach, dt, deter, det, U0, U1, U2, Grid_P, Q_C, metric, ROE, Fv, Ev, Gv, F, E, G, FLOW,
+             n_launch: int, params: Params) -> None:
+        _check(_lib.cfdx_weno(self.h, j, mach, dt, _ptr(deter), _ptr(det), _ptr(U0), _ptr(U1), _ptr(U2),
+                              _ptr(Grid_P), _ptr(Q_C), _ptr(metric), _ptr(ROE), _ptr(Fv), _ptr(Ev), _ptr(Gv),
+                              _ptr(F), _ptr(E), _ptr(G), _ptr(FLOW), n_launch, C.byref(params)))
+
+
+def _to_device(a: np.ndarray, device):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1))
+    return t.to(device)
+
+
+def _from_device(t, like: np.ndarray) -> np.ndarray:
+    return t.cpu().numpy().view(like.dtype).reshape(like.shape)
+
+
+class DropInSolver:
+    """Solver.cpp:356-541 on device: per stage U_Q -> Viscous -> Weno through the drop-in
+    kernels, with the USE_GPU==1 data contract (RK base U0 from U_Q at j = 0, raw stage
+    vectors U1 / U2 into the viscous kernel at j = 1, 2). Buffers stay in HBM between
+    stages; periodic single subdomain (no halo exchange / boundary fill)."""
+
+    def __init__(self, case: dict, ctx: Ctx | None = None, device: str = "cuda"):
+        import torch
+        self.ctx = ctx or Ctx()
+        self.n = case["Grid_P"].shape[0]
+        self.case = case
+        dev = torch.device(device)
+        self.Grid_P = _to_device(case["Grid_P"], dev)
+        self.det = torch.from_numpy(case["det"]).to(dev)
+        self.metric = torch.from_numpy(case["metric"]).to(dev)
+        self.deter = torch.from_numpy(case["deter"]).to(dev)
+        self.FLOW = _to_device(case["FLOW"], dev)
+        z = lambda k: torch.zeros((self.n, k), dtype=torch.float64, device=dev)
+        self.U0, self.U1, self.U2, self.Q_C = z(5), z(5), z(5), z(5)
+        self.F, self.E, self.G, self.Fv, self.Ev, self.Gv = (z(5) for _ in range(6))
+        self.Uscratch = z(5)
+        self.ROE = z(ROE_N)
+
+    def step(self, nsteps: int, mach: float, dt: float, params: Params) -> None:
+        n = self.n
+        c = self.ctx
+        for _ in range(nsteps):
+            for j in range(3):
+                c.uq(self.U0 if j == 0 else self.Uscratch, self.Q_C, self.Grid_P, self.det, self.FLOW, self.ROE, n)
+                uv = (self.U0, self.U1, self.U2)[j]
+                c.viscous(uv, self.F, self.E, self.G, self.Fv, self.Ev, self.Gv, self.Grid_P, self.metric, self.det,
+                          self.FLOW, n, params)
+                c.weno(j, mach, dt, self.deter, self.det, self.U0, self.U1, self.U2, self.Grid_P, self.Q_C,
+                       self.metric, self.ROE, self.Fv, self.Ev, self.Gv, self.F, self.E, self.G, self.FLOW, n, params)
+
+    def flow(self) -> np.ndarray:
+        self.ctx.sync()
+        return _from_device(self.FLOW, self.case["FLOW"])
+
+
+# ---------------------------------------------------------------------------
+# structured block (SoA, directional sweeps)
+# ---------------------------------------------------------------------------
+class Block:
+    """One periodic ni x nj x nk block resident in HBM (nodes in i-fastest order)."""
+
+    def __init__(self, ni: int, nj: int, nk: int, precision: int = FP64, ctx: Ctx | None = None):
+        self.ctx = ctx or Ctx()
+        self.desc = BlockDesc(ni, nj, nk, 7, precision)
+        h = _vp()
+        _check(_lib.cfdx_block_create(self.ctx.h, C.byref(self.desc), C.byref(h)))
+        self.h = h
+        self.n = ni * nj * nk
+
+    @staticmethod
+    def bytes_for(ni: int, nj: int, nk: int, precision: int = FP64) -> int:
+        d = BlockDesc(ni, nj, nk, 7, precision)
+        return int(_lib.cfdx_block_bytes(C.byref(d)))
+
+    def close(self) -> None:
+        if self.h:
+            _check(_lib.cfdx_block_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_case(self, case: dict) -> None:
+        """Geometry and state of a reference-numbered case (node 0 dropped)."""
+        det = np.ascontiguousarray(case["det"][1:])
+        metric = np.ascontiguousarray(case["metric"][1:])
+        deter = np.ascontiguousarray(case["deter"][1:])
+        _check(_lib.cfdx_block_set_geometry(self.h, _ptr(det), _ptr(metric), _ptr(deter)))
+        self.set_flow(case["FLOW"][1:])
+
+    def set_flow(self, flow: np.ndarray) -> None:
+        flow = np.ascontiguousarray(flow)
+        assert flow.dtype == FLOW_DT and flow.shape[0] == self.n
+        _check(_lib.cfdx_block_set_flow(self.h, _ptr(flow)))
+
+    def get_flow(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=FLOW_DT)
+        _check(_lib.cfdx_block_get_flow(self.h, _ptr(out)))
+        return out
+
+    def step(self, nsteps: int, mach: float, dt: float, params: Params) -> None:
+        _check(_lib.cfdx_block_step(self.h, mach, dt, nsteps, C.byref(params)))
+
+    def rhs(self, params: Params) -> np.ndarray:
+        out = np.zeros((5, self.n))
+        _check(_lib.cfdx_block_rhs(self.h, C.byref(params), _ptr(out)))
+        return out
+
+    def set_timing(self, on: bool) -> None:
+        _check(_lib.cfdx_block_set_timing(self.h, int(on)))
+
+    def timing(self) -> dict:
+        a, b, c, n = _d(), _d(), _d(), C.c_int()
+        _check(_lib.cfdx_block_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(n)))
+        return dict(visc_ms=a.value, face_ms=b.value, update_ms=c.value, launches=n.value)
+
+
+def exported_symbols() -> list[str]:
+    return list(EXPORTS)

@@ -1,0 +1,51 @@
+// Context (device + stream) and HIP error plumbing for the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+struct cfdx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+};
+
+namespace cfdx {
+
+inline int hip_fail(hipError_t e, const char *what)
+{
+    return fail(CFDX_E_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+// Makes ctx's device current; returns 0 or a CFDX error.
+inline int enter(cfdx_ctx *ctx)
+{
+    if (!ctx) return fail(CFDX_E_INVALID, "null context");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return CFDX_OK;
+}
+
+// Checks the launch that was just issued.
+inline int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return CFDX_OK;
+}
+
+inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace cfdx
+
+#define CFDX_TRY(expr)                 \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != CFDX_OK) return _rc; \
+    } while (0)
+
+#define CFDX_HIP(expr, what)                                  \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return cfdx::hip_fail(_e, what); \
+    } while (0)

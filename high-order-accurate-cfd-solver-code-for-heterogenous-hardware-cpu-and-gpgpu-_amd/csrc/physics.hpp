@@ -1,0 +1,330 @@
+// Device physics shared by the drop-in kernels and the structured sweeps.
+// T is the arithmetic type of fluxes / reconstruction / viscous terms (double for
+// the plain form, float for the mixed form). gamma = 1.4 as the reference's
+// literals 1.4 / 0.4 / 0.2 (Weno.cl:2292-2295, U_Q.cl:73).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cfdx {
+namespace dev {
+
+template <typename T> __device__ __forceinline__ T sqrt_(T x);
+template <> __device__ __forceinline__ double sqrt_<double>(double x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ float sqrt_<float>(float x) { return __builtin_sqrtf(x); }
+
+template <typename T> __device__ __forceinline__ T fabs_(T x) { return x < T(0) ? -x : x; }
+
+// Conserved state Q = (rho, rho u, rho v, rho w, p/0.4 + rho |u|^2 / 2) of primitives (U_Q.cl:118-122).
+template <typename T>
+__device__ __forceinline__ void prim_to_q(T rho, T u, T v, T w, T p, T (&q)[5])
+{
+    const T ke = T(0.5) * rho * (u * u + v * v + w * w);
+    q[0] = rho;
+    q[1] = rho * u;
+    q[2] = rho * v;
+    q[3] = rho * w;
+    q[4] = (p / T(0.4)) + ke;
+}
+
+// Total enthalpy H = (E + p) / rho of primitives (U_Q.cl:73-74).
+template <typename T>
+__device__ __forceinline__ T enthalpy(T rho, T u, T v, T w, T p)
+{
+    return ((p / T(0.4)) + T(0.5) * rho * (u * u + v * v + w * w) + p) / rho;
+}
+
+// Roe average of the face between states a and b (U_Q.cl:60-109): (u, v, w, h, rho).
+template <typename T>
+__device__ __forceinline__ void roe_average(T ra, T ua, T va, T wa, T ha, T rb, T ub, T vb, T wb, T hb, T (&r)[5])
+{
+    const T sa = sqrt_(ra), sb = sqrt_(rb);
+    const T is = T(1) / (sa + sb);
+    r[0] = (sb * ub + sa * ua) * is;
+    r[1] = (sb * vb + sa * va) * is;
+    r[2] = (sb * wb + sa * wa) * is;
+    r[3] = (sb * hb + sa * ha) * is;
+    r[4] = sb * sa;
+}
+
+// Contravariant, det-scaled inviscid node flux along metric row m (Viscous.cl:560-611):
+// D * (mx F1 + my E1 + mz G1) with F1, E1, G1 the Cartesian Euler fluxes.
+template <typename T>
+__device__ __forceinline__ void node_flux(T rho, T u, T v, T w, T p, T mx, T my, T mz, T D, T (&f)[5])
+{
+    const T V = mx * u + my * v + mz * w;
+    const T E = (p / T(0.4)) + T(0.5) * rho * (u * u + v * v + w * w);
+    const T rV = rho * V;
+    f[0] = D * rV;
+    f[1] = D * (rV * u + mx * p);
+    f[2] = D * (rV * v + my * p);
+    f[3] = D * (rV * w + mz * p);
+    f[4] = D * ((E + p) * V);
+}
+
+// WCNS-E5 interpolation of one characteristic field to the face between W[2] and W[3]
+// (W[0..5] at offsets -2..+3 from the left node), Weno.cl:1056-1279:
+//   candidates (15/8, -5/4, 3/8), (3/8, 3/4, -1/8), (-1/8, 3/4, 3/8),
+//   Jiang-Shu indicators, w_k = d_k / (eps + IS_k)^2, value = sum w_k q_k / sum w_k.
+// pos = right-biased (d = 1/16, 5/8, 5/16), neg = left-biased (d = 5/16, 5/8, 1/16).
+template <typename T>
+__device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg)
+{
+    const T c3o8 = T(3.0 / 8.0), c3o4 = T(3.0 / 4.0), c1o8 = T(1.0 / 8.0), c15o8 = T(15.0 / 8.0), c5o4 = T(5.0 / 4.0);
+    const T h_r = c15o8 * W[3] - c5o4 * W[4] + c3o8 * W[5];   // pos q0 (i+1, i+2, i+3)
+    const T h_c = c3o8 * W[2] + c3o4 * W[3] - c1o8 * W[4];    // pos q1 = neg q0 (i, i+1, i+2)
+    const T h_l = -c1o8 * W[1] + c3o4 * W[2] + c3o8 * W[3];   // pos q2 = neg q1 (i-1, i, i+1)
+    const T h_ll = c3o8 * W[0] - c5o4 * W[1] + c15o8 * W[2];  // neg q2 (i-2, i-1, i)
+    const T c13 = T(13.0 / 12.0), c14 = T(0.25);
+    const T s0 = W[0] - T(2) * W[1] + W[2];
+    const T s1 = W[1] - T(2) * W[2] + W[3];
+    const T s2 = W[2] - T(2) * W[3] + W[4];
+    const T s3 = W[3] - T(2) * W[4] + W[5];
+    // pos indicators
+    const T t0 = T(3) * W[3] - T(4) * W[4] + W[5];
+    const T t1 = W[2] - W[4];
+    const T t2 = W[1] - T(4) * W[2] + T(3) * W[3];
+    const T ISp0 = c13 * s3 * s3 + c14 * t0 * t0;
+    const T ISp1 = c13 * s2 * s2 + c14 * t1 * t1;
+    const T ISp2 = c13 * s1 * s1 + c14 * t2 * t2;
+    // neg indicators
+    const T u0 = T(3) * W[2] - T(4) * W[3] + W[4];
+    const T u1 = W[1] - W[3];
+    const T u2 = W[0] - T(4) * W[1] + T(3) * W[2];
+    const T ISm0 = c13 * s2 * s2 + c14 * u0 * u0;
+    const T ISm1 = c13 * s1 * s1 + c14 * u1 * u1;
+    const T ISm2 = c13 * s0 * s0 + c14 * u2 * u2;
+    {
+        T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
+        a0 *= a0; a1 *= a1; a2 *= a2;
+        // w_k proportional to d_k * prod_{m != k} a_m
+        const T w0 = T(1.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(5.0 / 16.0) * (a0 * a1);
+        pos = (w0 * h_r + w1 * h_c + w2 * h_l) / (w0 + w1 + w2);
+    }
+    {
+        T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
+        a0 *= a0; a1 *= a1; a2 *= a2;
+        const T w0 = T(5.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(1.0 / 16.0) * (a0 * a1);
+        neg = (w0 * h_c + w1 * h_l + w2 * h_ll) / (w0 + w1 + w2);
+    }
+}
+
+// Characteristic reconstruction of the two interface states of one face
+// (Weno.cl:336-455 eigenvectors and projection, :1056-2187 WCNS, :2188-2287 back projection).
+//   q[m][c]: conserved Q at offsets -2..+3 from the left node
+//   roe = (u, v, w, h, rho) of the face; k = unnormalised face metric row
+//   qp / qn: right-biased / left-biased conserved states (not det-scaled)
+template <typename T>
+__device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T eps,
+                                                      T (&qp)[5], T (&qn)[5])
+{
+    const T u = roe[0], v = roe[1], w = roe[2], h = roe[3], rho = roe[4];
+    const T ikk = T(1) / sqrt_(k0 * k0 + k1 * k1 + k2 * k2);
+    const T kx = k0 * ikk, ky = k1 * ikk, kz = k2 * ikk;
+    const T q2 = u * u + v * v + w * w;
+    const T a = sqrt_(T(0.4) * (h - T(0.5) * q2));
+    const T theta = kx * u + ky * v + kz * w;
+    const T phi = T(0.2) * q2;
+    const T irho = T(1) / rho;
+    const T ia = T(1) / a;
+    const T iaa = ia * ia;
+    const T s2 = T(0.70710678118654752440);           // 1/sqrt(2)
+    const T alpha = rho * ia * s2;                      // rho / (sqrt2 a)
+    const T beta = irho * ia * s2;                      // 1 / (sqrt2 rho a)
+    const T g4aa = T(0.4) * iaa;
+    const T one_m = T(1) - phi * iaa;
+    const T cv = kz * v - ky * w, cw = kx * w - kz * u, cu = ky * u - kx * v;  // n x u components
+    const T phi_g = phi / T(0.4);
+    const T hh = (phi + a * a) / T(0.4);
+
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { qp[m] = T(0); qn[m] = T(0); }
+
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        T l0, l1, l2, l3, l4;   // row c of L
+        T r0, r1, r2, r3, r4;   // column c of R
+        if (c == 0) {
+            l0 = kx * one_m - cv * irho; l1 = kx * g4aa * u; l2 = kx * g4aa * v + kz * irho; l3 = kx * g4aa * w - ky * irho; l4 = -kx * g4aa;
+            r0 = kx; r1 = kx * u; r2 = kx * v + kz * rho; r3 = kx * w - ky * rho; r4 = kx * phi_g + rho * cv;
+        } else if (c == 1) {
+            l0 = ky * one_m - cw * irho; l1 = ky * g4aa * u - kz * irho; l2 = ky * g4aa * v; l3 = ky * g4aa * w + kx * irho; l4 = -ky * g4aa;
+            r0 = ky; r1 = ky * u - kz * rho; r2 = ky * v; r3 = ky * w + kx * rho; r4 = ky * phi_g + rho * cw;
+        } else if (c == 2) {
+            l0 = kz * one_m - cu * irho; l1 = kz * g4aa * u + ky * irho; l2 = kz * g4aa * v - kx * irho; l3 = kz * g4aa * w; l4 = -kz * g4aa;
+            r0 = kz; r1 = kz * u + ky * rho; r2 = kz * v - kx * rho; r3 = kz * w; r4 = kz * phi_g + rho * cu;
+        } else if (c == 3) {
+            l0 = beta * (phi - theta * a); l1 = -beta * (T(0.4) * u - kx * a); l2 = -beta * (T(0.4) * v - ky * a);
+            l3 = -beta * (T(0.4) * w - kz * a); l4 = beta * T(0.4);
+            r0 = alpha; r1 = alpha * (u + kx * a); r2 = alpha * (v + ky * a); r3 = alpha * (w + kz * a); r4 = alpha * (hh + theta * a);
+        } else {
+            l0 = beta * (phi + theta * a); l1 = -beta * (T(0.4) * u + kx * a); l2 = -beta * (T(0.4) * v + ky * a);
+            l3 = -beta * (T(0.4) * w + kz * a); l4 = beta * T(0.4);
+            r0 = alpha; r1 = alpha * (u - kx * a); r2 = alpha * (v - ky * a); r3 = alpha * (w - kz * a); r4 = alpha * (hh - theta * a);
+        }
+        T W[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) W[m] = l0 * q[m][0] + l1 * q[m][1] + l2 * q[m][2] + l3 * q[m][3] + l4 * q[m][4];
+        T cp, cn;
+        wcns_pair(W, eps, cp, cn);
+        qp[0] += r0 * cp; qp[1] += r1 * cp; qp[2] += r2 * cp; qp[3] += r3 * cp; qp[4] += r4 * cp;
+        qn[0] += r0 * cn; qn[1] += r1 * cn; qn[2] += r2 * cn; qn[3] += r3 * cn; qn[4] += r4 * cn;
+    }
+}
+
+// det_f * (k . (F, E, G))(Q) and lambda_5 = k.u + c |k| of one interface state
+// (Weno.cl:2291-2507 fluxes, :2511-2581 eigenvalues, :2585-2655 contravariant form).
+template <typename T>
+__device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k2, T kk2, T detf, T (&c)[5], T &lam)
+{
+    const T r = T(1) / Q[0];
+    const T ke = (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * r;
+    const T P = T(0.4) * (Q[4] - T(0.5) * ke);
+    const T Un = (k0 * Q[1] + k1 * Q[2] + k2 * Q[3]) * r;
+    c[0] = detf * (Q[0] * Un);
+    c[1] = detf * (Q[1] * Un + k0 * P);
+    c[2] = detf * (Q[2] * Un + k1 * P);
+    c[3] = detf * (Q[3] * Un + k2 * P);
+    c[4] = detf * ((T(1.4) * Q[4] - T(0.2) * ke) * Un);
+    lam = Un + sqrt_(T(1.4) * P * r * kk2);
+}
+
+// Face flux: LLF with alpha = max(|lambda_5|) (Weno.cl:2859-2947) plus the explicit
+// midpoint correction - delta2 F / 24 + 7 delta4 F / 5760 from node fluxes nf at offsets
+// -2..+3 (Weno.cl:2951-3058). Output is F-hat of the face (det-scaled, contravariant).
+template <typename T, typename TO>
+__device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
+                                          const T (&nf)[6][5], TO (&out)[5])
+{
+    T qp[5], qn[5];
+    characteristic_states(q, roe, k0, k1, k2, eps, qp, qn);
+    const T kk2 = k0 * k0 + k1 * k1 + k2 * k2;
+    T cp[5], cn[5], lp, ln;
+    face_state_flux(qp, k0, k1, k2, kk2, detf, cp, lp);
+    face_state_flux(qn, k0, k1, k2, kk2, detf, cn, ln);
+    const T al = fabs_(lp) >= fabs_(ln) ? fabs_(lp) : fabs_(ln);
+    const T ad = al * detf;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const T llf = T(0.5) * (cp[c] + cn[c] - ad * (qp[c] - qn[c]));
+        const T d2 = T(1.0 / 48.0) * (T(-5) * nf[0][c] + T(39) * nf[1][c] - T(34) * nf[2][c] - T(34) * nf[3][c] + T(39) * nf[4][c] - T(5) * nf[5][c]);
+        const T d4 = T(0.5) * (nf[0][c] - T(3) * nf[1][c] + T(2) * nf[2][c] + T(2) * nf[3][c] - T(3) * nf[4][c] + nf[5][c]);
+        out[c] = TO(llf - T(1.0 / 24.0) * d2 + T(7.0 / 5760.0) * d4);
+    }
+}
+
+// Viscous + MTS-SGS node fluxes (Viscous.cl:439-631) of one node.
+//   dU[c][d] = dU_c / d(zeta, eta, xi)_d of the det-scaled state U (Viscous.cl:285-435)
+//   U[5]     = U at the node; u, v, w = FLOW velocities at the node (Viscous.cl:146-154)
+//   m[9]     = node metrics zeta_x..xi_z; D = det
+//   KS[3]    = MTS scale-similarity terms KS_x, KS_y, KS_z (Viscous.cl:493-495)
+// Writes the contravariant det-scaled viscous fluxes fvis[0..2] = Fv, Ev, Gv and returns mu.
+struct ViscParams { double mach, reyl, pr, free_t, c_mts, c_t; };
+
+template <typename T> __device__ __forceinline__ T cbrt_(T x);
+template <> __device__ __forceinline__ double cbrt_<double>(double x) { return cbrt(x); }
+template <> __device__ __forceinline__ float cbrt_<float>(float x) { return cbrtf(x); }
+
+template <typename T>
+__device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], T u, T v, T w, const T (&m)[9], T D,
+                                          const T (&KS)[3], const ViscParams &vp, T (&fvis)[3][5])
+{
+    const T Mach = T(vp.mach), Reyl = T(vp.reyl), Pr = T(vp.pr), Free_t = T(vp.free_t);
+    const T C_MTS = T(vp.c_mts), C_T = T(vp.c_t);
+    const T iU0 = T(1) / U[0];
+    const T iU02 = iU0 * iU0;
+    T ug[3], vg[3], wg[3], tg[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        ug[d] = iU0 * dU[1][d] - (U[1] * iU02) * dU[0][d];
+        vg[d] = iU0 * dU[2][d] - (U[2] * iU02) * dU[0][d];
+        wg[d] = iU0 * dU[3][d] - (U[3] * iU02) * dU[0][d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        tg[d] = iU0 * dU[4][d] - (U[4] * iU02) * dU[0][d] - (U[1] * iU0) * ug[d] - (U[2] * iU0) * vg[d] - (U[3] * iU0) * wg[d];
+    // Cartesian gradients: X_x = X_zeta zeta_x + X_eta eta_x + X_xi xi_x, etc.
+    const T ux = ug[0] * m[0] + ug[1] * m[3] + ug[2] * m[6];
+    const T uy = ug[0] * m[1] + ug[1] * m[4] + ug[2] * m[7];
+    const T uz = ug[0] * m[2] + ug[1] * m[5] + ug[2] * m[8];
+    const T vx = vg[0] * m[0] + vg[1] * m[3] + vg[2] * m[6];
+    const T vy = vg[0] * m[1] + vg[1] * m[4] + vg[2] * m[7];
+    const T vz = vg[0] * m[2] + vg[1] * m[5] + vg[2] * m[8];
+    const T wx = wg[0] * m[0] + wg[1] * m[3] + wg[2] * m[6];
+    const T wy = wg[0] * m[1] + wg[1] * m[4] + wg[2] * m[7];
+    const T wz = wg[0] * m[2] + wg[1] * m[5] + wg[2] * m[8];
+    const T ex = tg[0] * m[0] + tg[1] * m[3] + tg[2] * m[6];
+    const T ey = tg[0] * m[1] + tg[1] * m[4] + tg[2] * m[7];
+    const T ez = tg[0] * m[2] + tg[1] * m[5] + tg[2] * m[8];
+
+    // Viscous.cl:455-458: T = gamma M^2 (gamma-1) e, Sutherland mu with S = 110.4
+    const T d_temp = T(1.4) * Mach * Mach * T(0.4) * (U[4] * iU0 - T(0.5) * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * iU02));
+    const T st = sqrt_(d_temp);
+    const T S = T(110.4) / Free_t;
+    const T mu = (st * st * st) * ((T(1) + S) / (d_temp + S));
+    // Viscous.cl:461-476 heat flux and Stokes stresses
+    const T kq = -(mu * T(1.4) / (Pr * Reyl));
+    const T qx = kq * ex, qy = kq * ey, qz = kq * ez;
+    const T mr = mu / Reyl;
+    const T txx = mr * (T(4.0 / 3.0) * ux - T(2.0 / 3.0) * vy - T(2.0 / 3.0) * wz);
+    const T tyy = mr * (T(4.0 / 3.0) * vy - T(2.0 / 3.0) * ux - T(2.0 / 3.0) * wz);
+    const T tzz = mr * (T(4.0 / 3.0) * wz - T(2.0 / 3.0) * ux - T(2.0 / 3.0) * vy);
+    const T txy = mr * (uy + vx);
+    const T txz = mr * (uz + wx);
+    const T tyz = mr * (vz + wy);
+    // Viscous.cl:490-558 mixed-time-scale eddy viscosity, one per direction
+    const T sxy = T(0.5) * (uy + vx), syz = T(0.5) * (vz + wy), sxz = T(0.5) * (wx + uz);
+    const T sxx = ux, syy = vy, szz = wz;
+    const T mod = sqrt_(T(2) * ((sxy * sxy + syz * syz + sxz * sxz) + T(2.0 / 3.0) * (sxx * sxx + syy * syy + szz * szz)));
+    const T DEL = T(1) / cbrt_(D);
+    T eddy[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T sk = sqrt_(KS[d]);
+        if (mod != T(0)) {
+            const T R = (DEL * mod) / sk;
+            eddy[d] = (C_MTS / (T(1) + R / C_T)) * DEL * sk;
+        } else {
+            eddy[d] = C_MTS * DEL * sk;
+        }
+    }
+    // SGS stresses scale with U0 = det * rho (Viscous.cl:529-535)
+    const T m2U = T(-2) * U[0];
+    const T Sxy = m2U * eddy[0] * sxy, Syz = m2U * eddy[1] * syz, Sxz = m2U * eddy[2] * sxz;
+    const T Sxx = m2U * eddy[0] * sxx, Syy = m2U * eddy[1] * syy, Szz = m2U * eddy[2] * szz;
+    const T kQ = -(T(1) / (T(1.4) * T(0.4) * Mach * Mach));
+    const T Qx = kQ * eddy[0] * ex, Qy = kQ * eddy[1] * ey, Qz = kQ * eddy[2] * ez;
+    const T DX = (Sxx + Sxy + Sxz) * u, DY = (Sxy + Syy + Syz) * v, DZ = (Sxz + Syz + Szz) * w;
+    // H = Q + DEL and D = 0.4 Q (Viscous.cl:549-557)
+    const T Hx = Qx + DX + T(0.4) * Qx, Hy = Qy + DY + T(0.4) * Qy, Hz = Qz + DZ + T(0.4) * Qz;
+    // Viscous.cl:576-595 Cartesian viscous fluxes
+    const T tF[5] = {T(0), -txx + Sxx, -txy + Sxy, -txz + Sxz, (-u * txx - v * txy - w * txz) + qx + Hx};
+    const T tE[5] = {T(0), -txy + Sxy, -tyy + Syy, -tyz + Syz, (-u * txy - v * tyy - w * tyz) + qy + Hy};
+    const T tG[5] = {T(0), -txz + Sxz, -tyz + Syz, -tzz + Szz, (-u * txz - v * tyz - w * tzz) + qz + Hz};
+    // Viscous.cl:615-631 contravariant and det-scaled
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) fvis[d][c] = D * (m[3 * d] * tF[c] + m[3 * d + 1] * tE[c] + m[3 * d + 2] * tG[c]);
+    return mu;
+}
+
+// One-direction derivative of Viscous.cl:285-435 (weno_form = false) or Weno.cl:2732-2856
+// (weno_form = true) selected by the loc codes; f[0..6] at offsets -3..+3. Branches are
+// taken in source order, a later match overwriting an earlier one; with no match the
+// reference leaves the value undefined and this returns 0.
+template <typename T>
+__device__ __forceinline__ T loc_derivative(const T (&f)[7], int LI, int LM2, int LM1, int LP1, int LP2, bool weno_form)
+{
+    T d = T(0);
+    if (LI > 0 && LI <= 52 && LP1 == 100) d = T(1.0 / 6.0) * (T(11) * f[3] - T(18) * f[2] + T(9) * f[1] - T(2) * f[0]);
+    if (LI > 0 && LI <= 52 && LM1 == 100) d = T(1.0 / 6.0) * (T(-11) * f[3] + T(18) * f[4] - T(9) * f[5] + T(2) * f[6]);
+    if (LI == 0 && LP1 > 0 && LP1 <= 52 && LP2 == 100) d = T(1.0 / 6.0) * (T(2) * f[4] + T(3) * f[3] - T(6) * f[2] + f[1]);
+    if (LI == 0 && LM1 > 0 && (weno_form ? LM1 <= 52 : true) && LM2 == 100)
+        d = T(1.0 / 6.0) * (T(-2) * f[2] - T(3) * f[3] + T(6) * f[4] - f[5]);
+    if (LM2 != 100 && LM1 != 100 && LP2 != 100 && LP1 != 100 && (LI == 0 || (!weno_form && LI > 0)))
+        d = T(1.0 / 12.0) * (T(8) * (f[4] - f[2]) - (f[5] - f[1]));
+    return d;
+}
+
+}  // namespace dev
+}  // namespace cfdx

@@ -1,0 +1,198 @@
+/*
+ * cfdx — MI355X (gfx950) implementation of the per-RK-stage right-hand side and
+ * time update of the high-order curvilinear compressible Navier-Stokes / LES
+ * solver (reference kernels U_Q.cl, Viscous.cl, Weno.cl driven by Solver.cpp).
+ *
+ * C ABI only: plain pointers and sizes, no C++ or torch types. Every function
+ * returns 0 on success or a negative CFDX_E_* code; cfdx_last_error() gives
+ * the message of the last failure on the calling thread. Nothing here exits
+ * the process.
+ *
+ * Three layers:
+ *  1. Drop-in kernels (cfdx_uq / cfdx_viscous / cfdx_weno): the reference's
+ *     three OpenCL kernels with the same argument order, packed AoS layouts and
+ *     per-node semantics, on caller-owned DEVICE buffers. n_launch is the
+ *     reference's global size (g_node+1 or sd_node+1; index 0 is the dummy node).
+ *  2. Structured blocks (cfdx_block_*): one i,j,k block resident in HBM in
+ *     structure-of-arrays form, stepped by directional sweep kernels.
+ *  3. Host geometry (cfdx_geom_*): neighbour tables, node metrics, half-point
+ *     metrics and initial states that feed either layer (CPU only).
+ */
+#ifndef CFDX_H
+#define CFDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Error codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define CFDX_OK             0
+#define CFDX_E_INVALID     -1   /* bad argument (null pointer, size, flag) */
+#define CFDX_E_HIP         -2   /* HIP runtime call failed */
+#define CFDX_E_NOMEM       -3   /* device allocation failed */
+#define CFDX_E_UNSUPPORTED -4   /* valid request this build does not implement */
+#define CFDX_E_NONFINITE   -5   /* non-finite state detected by a checked step */
+#define CFDX_E_CAPACITY    -6   /* request exceeds a fixed capacity */
+
+const char *cfdx_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Packed layouts, byte-identical to functions.h:49-112 of the reference     */
+/* ------------------------------------------------------------------------ */
+
+/* FLOW_VARIABLES (functions.h:49-51), 72 bytes */
+typedef struct __attribute__((packed)) { double u, v, w, p, t, rho, mu, a, e; } cfdx_flow;
+/* Solver_Column_matrix (functions.h:53-55), 40 bytes */
+typedef struct __attribute__((packed)) { double N0, N1, N2, N3, N4; } cfdx_vec5;
+/* GRID (functions.h:57-61), 56 bytes: N1 = +i, N3 = -i, N0 = +j, N2 = -j, N4 = +k, N5 = -k;
+ * loc = 0 interior, 1..52 boundary kinds, 100 ghost (Main.cpp:1511, node_locations.cpp). */
+typedef struct __attribute__((packed)) { double x, y, z; int32_t N0, N1, N2, N3, N4, N5, ID, loc; } cfdx_grid;
+/* ROE_AVERAGING (functions.h:63-70), 288 bytes: faces ip, jp, kp, im, jm, km, each (u, v, w, p, h, rho) */
+typedef struct __attribute__((packed)) { double f[6][6]; } cfdx_roe;
+/* TRANSFORMED (functions.h:93-98), 504 bytes: node metrics zeta_x..xi_z, then per face
+ * ip, im, jp, jm, kp, km the same nine metrics interpolated to the half point */
+typedef struct __attribute__((packed)) { double node[9]; double face[6][9]; } cfdx_metric;
+/* DETERM (functions.h:110-112), 48 bytes: half-point det at ip, im, jp, jm, kp, km */
+typedef struct __attribute__((packed)) { double ip, im, jp, jm, kp, km; } cfdx_deter;
+
+/* ------------------------------------------------------------------------ */
+/* Physical / numerical parameters                                           */
+/* ------------------------------------------------------------------------ */
+#define CFDX_FP64  0   /* all arithmetic in double */
+#define CFDX_MIXED 1   /* fluxes, reconstruction, projection, viscous terms in float;
+                          state, RK accumulation, metrics and norms in double */
+
+typedef struct {
+    double mach;      /* Viscous.cl:30 — Mach number in T and the SGS heat flux (ref 1.5) */
+    double reyl;      /* Viscous.cl:31 — Reynolds number; INFINITY = inviscid stresses (ref 500) */
+    double pr;        /* Viscous.cl:32 — Prandtl number (ref 0.72) */
+    double free_t;    /* Viscous.cl:33 — Sutherland reference temperature (ref 273) */
+    double c_mts;     /* Viscous.cl:490 — mixed-time-scale constant; 0 disables SGS (ref 0.03) */
+    double c_t;       /* Viscous.cl:491 — MTS time-scale constant (ref 10) */
+    double eps;       /* Weno.cl:59 — WCNS epsilon "10e-8" (ref 1e-7) */
+    int32_t precision;/* CFDX_FP64 or CFDX_MIXED (structured blocks only) */
+    int32_t viscous;  /* 1 = full RHS; 0 = Euler (only when reyl is infinite and c_mts == 0) */
+} cfdx_params;
+
+/* Fills the reference's constants, precision FP64, viscous on. */
+void cfdx_params_default(cfdx_params *p);
+
+/* ------------------------------------------------------------------------ */
+/* Context: one device + one HIP stream                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct cfdx_ctx cfdx_ctx;
+
+/* device: HIP ordinal. stream: a hipStream_t to launch on, or NULL for a
+ * stream owned by the context. */
+int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out);
+int cfdx_ctx_destroy(cfdx_ctx *ctx);
+int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream);
+void *cfdx_ctx_stream(cfdx_ctx *ctx);
+int cfdx_sync(cfdx_ctx *ctx);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in kernels (device pointers, reference argument order)           */
+/* ------------------------------------------------------------------------ */
+
+/* U_Q.cl:36-38 (arguments set at Solver.cpp:277-282). */
+int cfdx_uq(cfdx_ctx *ctx, cfdx_vec5 *U_C, cfdx_vec5 *Q_C, const cfdx_grid *Grid_P, const double *det,
+            cfdx_flow *FLOW, cfdx_roe *ROE_AVER, int64_t n_launch);
+
+/* Viscous.cl:23-26 (arguments set at Solver.cpp:284-294). The constants the
+ * reference hard-codes come from params (NULL = reference values). */
+int cfdx_viscous(cfdx_ctx *ctx, const cfdx_vec5 *U_C, cfdx_vec5 *F_C, cfdx_vec5 *E_C, cfdx_vec5 *G_C,
+                 cfdx_vec5 *Fv_C, cfdx_vec5 *Ev_C, cfdx_vec5 *Gv_C, const cfdx_grid *Grid_P,
+                 const cfdx_metric *metric, const double *det, cfdx_flow *FLOW, int64_t n_launch,
+                 const cfdx_params *params);
+
+/* Weno.cl:36-40 (arguments set at Solver.cpp:296-314; note F, E, G come after Fv, Ev, Gv). */
+int cfdx_weno(cfdx_ctx *ctx, int j, double Mach, double del_t, const cfdx_deter *deter, const double *det,
+              cfdx_vec5 *U0, cfdx_vec5 *U1, cfdx_vec5 *U2, const cfdx_grid *Grid_P, const cfdx_vec5 *Q_C,
+              const cfdx_metric *metric, const cfdx_roe *ROE_AVER, const cfdx_vec5 *Fv, const cfdx_vec5 *Ev,
+              const cfdx_vec5 *Gv, const cfdx_vec5 *F, const cfdx_vec5 *E, const cfdx_vec5 *G, cfdx_flow *FLOW,
+              int64_t n_launch, const cfdx_params *params);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Structured blocks                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct cfdx_block cfdx_block;
+
+typedef struct {
+    int32_t ni, nj, nk;   /* nodes along i (zeta), j (eta), k (xi); node (i,j,k) is i + ni*(j + nj*k) */
+    int32_t periodic;     /* must be 7 (periodic in i, j and k) */
+    int32_t precision;    /* CFDX_FP64 or CFDX_MIXED */
+} cfdx_block_desc;
+
+/* Bytes of device memory a block of this shape holds. */
+int64_t cfdx_block_bytes(const cfdx_block_desc *desc);
+int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfdx_block **out);
+int cfdx_block_destroy(cfdx_block *b);
+
+/* Host arrays in node order (n = ni*nj*nk entries, reference AoS layouts). */
+int cfdx_block_set_geometry(cfdx_block *b, const double *det, const cfdx_metric *metric, const cfdx_deter *deter);
+int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow);
+int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow);
+
+/* nsteps SSP-RK3 steps with fixed del_t; Mach is the Weno-argument Mach (t = gamma M^2 p/rho).
+ * Asynchronous on the context stream. */
+int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nsteps, const cfdx_params *params);
+
+/* The stage-0 right-hand side L (Weno.cl:3078-3082) of the current state, n*5 doubles
+ * (component-major: L[c*n + node]) copied to the host. Synchronous. */
+int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *L_host);
+
+/* Device time in ms of the kernels of the last cfdx_block_step, per kernel class,
+ * measured with HIP events on the context stream when timing is on. */
+int cfdx_block_set_timing(cfdx_block *b, int on);
+int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face, double *ms_update, int *launches);
+
+/* ------------------------------------------------------------------------ */
+/* 3. Host geometry (CPU)                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Neighbour table of a periodic ni x nj x nk block in the reference numbering:
+ * node 0 is the dummy node, node (i,j,k) is 1 + i + ni*(j + nj*k), all loc = 0.
+ * Grid_P must hold ni*nj*nk + 1 entries; x,y,z are taken from xyz (3 per node, may be NULL). */
+int cfdx_geom_periodic_table(int32_t ni, int32_t nj, int32_t nk, const double *xyz, cfdx_grid *Grid_P);
+
+/* Node coordinates, 3 doubles per node, node order i fastest. */
+int cfdx_geom_coords_box(int32_t ni, int32_t nj, int32_t nk, const double length[3], double *xyz);
+/* x = s1 + a sin(s2+px) sin(s3+px), y = s2 + a sin(s3+py) sin(s1+py), z = s3 + a sin(s1+pz) sin(s2+pz),
+ * s = (i, j, k) * length / n, a = amp. */
+int cfdx_geom_coords_sinusoid(int32_t ni, int32_t nj, int32_t nk, const double length[3], double amp,
+                              const double phase[3], double *xyz);
+
+/* Node Jacobian d(x,y,z)/d(i,j,k) (9 per node: x_zeta, x_eta, x_xi, y_zeta, ..., z_xi) of the two
+ * analytic maps above (computational spacing 1, Main.cpp:35-36). */
+int cfdx_geom_jacobian_sinusoid(int32_t ni, int32_t nj, int32_t nk, const double length[3], double amp,
+                                const double phase[3], double *jac);
+/* Node Jacobian from coordinates by the 6th-order compact scheme of Grid_Transformation.cpp:45-53
+ * along periodic grid lines (cyclic tridiagonal solve). */
+int cfdx_geom_jacobian_compact_periodic(int32_t ni, int32_t nj, int32_t nk, const double length[3],
+                                        const double *xyz, double *jac);
+/* det and node metrics from the Jacobian (Grid_Transformation.cpp:447, :1029-1037); the face
+ * metrics of metric[] are left untouched. n nodes, entries 0..n-1. */
+int cfdx_geom_node_metrics(int64_t n, const double *jac, double *det, cfdx_metric *metric);
+/* Half-point metrics and dets, Main.cpp:2368-2435, for nodes 1..n_sd of a reference-numbered table. */
+int cfdx_geom_half_metrics(const cfdx_grid *Grid_P, int64_t n_sd, cfdx_metric *metric, const double *det,
+                           cfdx_deter *deter);
+
+/* Initial states on reference-numbered nodes 1..n (node 0 gets the free stream).
+ * Free stream (Flow_Initialisation.cpp:35-48): u=1, v=w=0, p=1/(gamma M^2), T=1. */
+int cfdx_geom_init_freestream(int64_t n, double mach, cfdx_flow *flow);
+/* Isentropic vortex of strength beta centred at (xc, yc), advected with u = 1 (in free-stream units). */
+int cfdx_geom_init_vortex(int64_t n, const double *xyz, double mach, double beta, double xc, double yc,
+                          cfdx_flow *flow);
+/* Taylor-Green vortex: u = sin x cos y cos z, v = -cos x sin y cos z, w = 0,
+ * p = 1/(gamma M^2) + (cos 2x + cos 2y)(cos 2z + 2)/16, T = 1. */
+int cfdx_geom_init_tgv(int64_t n, const double *xyz, double mach, cfdx_flow *flow);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFDX_H */
