@@ -1,0 +1,182 @@
+"""Throughput of the per-step RHS + SSP-RK3 update on one MI355X.
+
+Workload (default): 128^3 periodic curvilinear Taylor-Green vortex, FP64 Navier-Stokes
+with the reference's mixed-time-scale LES model, Mach 0.1, Re 1600, CFL 0.5 (dt fixed
+from the initial state), full RHS (characteristic WCNS + LLF + 6th-order correction +
+viscous/SGS fluxes) and RK3. A step = one full SSP-RK3 step over the whole grid with
+the state resident in HBM. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "high-order-accurate-cfd-solver-code-for-heterogenous-hardware-cpu-and-gpgpu-_amd")
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+ALG_BYTES_PER_CELL_UPDATE = 848  # minimal FP64 SoA traffic of one RK3 step (3 stages)
+FACE_BYTES_PER_NODE = 144        # k_face: reads P(5) + metric row(3) + det + face k(3) + face det, writes F-hat(5)
+
+
+def _import_pkg():
+    spec = importlib.util.spec_from_file_location("cfdx", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["cfdx"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def workload(name: str):
+    import numpy as np
+    rng = np.random.default_rng(20261202)
+    if name in ("c2", "ns256"):
+        n = 128 if name == "c2" else 256
+        h = 2 * math.pi / n
+        phases = tuple(rng.uniform(0, 2 * math.pi, 3))
+        spec = dict(workload=f"taylor_green_les_{n}^3_curvilinear", grid=[n, n, n], mach=0.1, re=1600.0,
+                    sgs="mixed_time_scale(C=0.03,C_T=10)", cfl=0.5, map="sinusoidal amp=0.05h seeded phases")
+        case = dict(ni=n, nj=n, nk=n, kw=dict(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * h,
+                                               phases=phases, init="tgv", mach=0.1))
+        return spec, case, dict(mach=0.1, reyl=1600.0), True
+    if name == "c1":
+        spec = dict(workload="isentropic_vortex_euler_64^3", grid=[64, 64, 64], mach=0.5, beta=5.0, cfl=0.5)
+        case = dict(ni=64, nj=64, nk=64, kw=dict(grid="box", length=(10.0,) * 3, init="vortex", mach=0.5))
+        return spec, case, dict(), False
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(cfdx, max_seconds: float = 20.0):
+    """The reference's own kernels (oracle/_ref, gcc -O3, OpenMP) or the restatement, on a
+    bounded sample of the same workload: a 48^3 block of the curvilinear TGV, whole RK3 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    kind = "reference" if pyoracle.available("reference") else "port"
+    if not pyoracle.available(kind):
+        return None
+    orc = pyoracle.Oracle(kind)
+    cores = len(os.sched_getaffinity(0))
+    orc.set_threads(cores)
+    n = 48
+    h = 2 * math.pi / n
+    case = cfdx.periodic_case(n, n, n, grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * h,
+                              phases=(0.3, 1.1, 2.0), init="tgv", mach=0.1)
+    prm = pyoracle.reference_params(mach=0.1, reyl=1600.0)
+    dt = cfdx.cfl_dt(case, 0.5)
+    work = orc.new_work(case["Grid_P"].shape[0])
+    t0 = time.perf_counter()
+    orc.steps(case, 1, 0.1, dt, prm, work)
+    t1 = time.perf_counter()
+    steps = max(1, int(max_seconds / max(t1 - t0, 1e-6)) - 1)
+    steps = min(steps, 50)
+    t2 = time.perf_counter()
+    orc.steps(case, steps, 0.1, dt, prm, work)
+    t3 = time.perf_counter()
+    cups = n ** 3 * steps / (t3 - t2)
+    return {"value": cups, "unit": "cell-updates/s", "cores": cores, "kind": kind,
+            "sample": f"{steps} RK3 steps of a {n}^3 curvilinear TGV block (same kernels, FP64, OpenMP over nodes)",
+            "seconds": round(t3 - t2, 2)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the periodic block is single-GPU, multi-GPU halo exchange is design-only)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    cfdx = _import_pkg()
+    spec, cdef, pkw, viscous = workload(args.workload)
+    t_setup = time.perf_counter()
+    case = cfdx.periodic_case(cdef["ni"], cdef["nj"], cdef["nk"], **cdef["kw"])
+    prm = cfdx.default_params(**pkw) if viscous else cfdx.euler_params()
+    mach = cdef["kw"]["mach"]
+    dt = cfdx.cfl_dt(case, 0.5)
+    n = case["n"]
+    block = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"])
+    block.set_case(case)
+    del case
+    print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    ctx = block.ctx
+    block.step(args.warmup, mach, dt, prm)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.record(0)
+    block.step(args.steps, mach, dt, prm)
+    ctx.record(1)
+    ctx.sync()
+    t1 = time.perf_counter()
+    wall_s = t1 - t0
+    dev_ms = ctx.elapsed_ms(0, 1)
+    ms_per_step = dev_ms / args.steps
+    value = n * args.steps / (dev_ms / 1e3)
+    print(f"bench: {args.steps} steps wall {wall_s:.3f}s device {dev_ms:.1f}ms", file=sys.stderr, flush=True)
+
+    # per-kernel-class device time (HIP events on the block's stream, separate pass)
+    block.set_timing(True)
+    block.step(args.profile_steps, mach, dt, prm)
+    tk = block.timing()
+    block.set_timing(False)
+    face_launches = 9 * args.profile_steps
+    face_ms = tk["face_ms"] / face_launches
+    achieved = FACE_BYTES_PER_NODE * n / (face_ms / 1e3) / 1e9
+    flow = block.get_flow()
+    finite = bool(np.isfinite(cfdx_matrix(flow)).all())
+
+    line = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": dict(spec, parallelism="single GPU, one periodic block", precision="fp64"),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_face (one of 3 per stage)", "kernel_ms": face_ms,
+                     "bytes_per_launch": FACE_BYTES_PER_NODE * n},
+        "hbm_alg_frac_whole_step": ALG_BYTES_PER_CELL_UPDATE * value / (HBM_PEAK_GBS * 1e9),
+        "kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in tk.items() if k.endswith("_ms")},
+        "state_finite": finite,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(cfdx)
+        if cb:
+            line["cpu_baseline"] = cb
+            line["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(line))
+    return 0
+
+
+def cfdx_matrix(flow):
+    import numpy as np
+    return np.stack([flow[f] for f in ("rho", "u", "v", "w", "p")], axis=1)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -2,9 +2,13 @@
 CFD solver, behind the reference's kernel interface (U_Q / Viscous / Weno).
 
 Host-side mirror of the reference's Solver.cpp (stage sequence and argument
-layout) over the C ABI of libcfdx.so (include/cfdx.h). PyTorch is used only
-for device memory and streams; every computation runs in the HIP kernels of
-libcfdx.so, and importing this package fails loudly when that library is missing.
+layout) over the C ABI of libcfdx.so (include/cfdx.h). Device memory, copies,
+streams and events go through that library too; every computation runs in its
+HIP kernels, and importing this package fails loudly when the library is missing.
+
+Do not import torch in a process that uses this package on the GPU: torch's
+wheel ships its own HIP runtime under another file name, and two HIP runtimes
+in one process do not share the device.
 """
 from __future__ import annotations
 
@@ -67,6 +71,13 @@ EXPORTS = {
     "cfdx_ctx_set_stream": (C.c_int, [_vp, _vp]),
     "cfdx_ctx_stream": (_vp, [_vp]),
     "cfdx_sync": (C.c_int, [_vp]),
+    "cfdx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "cfdx_device_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "cfdx_device_free": (C.c_int, [_vp, _vp]),
+    "cfdx_copy_to_device": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "cfdx_copy_to_host": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "cfdx_event_record": (C.c_int, [_vp, C.c_int]),
+    "cfdx_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_d)]),
     "cfdx_uq": (C.c_int, [_vp] * 7 + [_i64]),
     "cfdx_viscous": (C.c_int, [_vp] * 12 + [_i64, C.POINTER(Params)]),
     "cfdx_weno": (C.c_int, [_vp, C.c_int, _d, _d] + [_vp] * 16 + [_i64, C.POINTER(Params)]),
@@ -104,12 +115,18 @@ def _check(rc: int) -> None:
 
 
 def _ptr(a) -> C.c_void_p:
-    """Address of a numpy array or a torch tensor (host or device)."""
+    """Address of a numpy array (host) or a DeviceArray (device)."""
     if a is None:
         return C.c_void_p(0)
     if isinstance(a, np.ndarray):
         return C.c_void_p(a.ctypes.data)
-    return C.c_void_p(a.data_ptr())
+    return C.c_void_p(a.ptr)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = _lib.cfdx_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
 
 
 def default_params(**kw) -> Params:
@@ -208,21 +225,16 @@ def cfl_dt(case: dict, cfl: float) -> float:
 
 
 # ---------------------------------------------------------------------------
-# context + drop-in kernels on device buffers (torch tensors)
+# context + drop-in kernels on device buffers
 # ---------------------------------------------------------------------------
 class Ctx:
-    """A device + stream. stream=None uses torch's current stream when torch is importable."""
+    """A device + stream (owned by the context unless a hipStream_t address is given)."""
 
-    def __init__(self, device: int = 0, stream=None):
+    def __init__(self, device: int = 0, stream: int | None = None):
         h = _vp()
-        if stream is None:
-            try:
-                import torch
-                stream = torch.cuda.current_stream(device).cuda_stream
-            except Exception:
-                stream = None
         _check(_lib.cfdx_ctx_create(device, _vp(stream) if stream else None, C.byref(h)))
         self.h = h
+        self.device = device
 
     def sync(self) -> None:
         _check(_lib.cfdx_sync(self.h))
@@ -238,7 +250,25 @@ class Ctx:
         except Exception:
             pass
 
-    # reference argument order (Solver.cpp:277-314)
+    def record(self, slot: int) -> None:
+        _check(_lib.cfdx_event_record(self.h, slot))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = _d()
+        _check(_lib.cfdx_event_elapsed(self.h, a, b, C.byref(ms)))
+        return ms.value
+
+    # device buffers
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.nbytes, a.dtype, a.shape)
+        _check(_lib.cfdx_copy_to_device(self.h, _vp(d.ptr), _ptr(a), a.nbytes))
+        return d
+
+    def zeros(self, shape, dtype=np.float64) -> "DeviceArray":
+        return self.to_device(np.zeros(shape, dtype=dtype))
+
+    # reference argument order (Solver.cpp:277-314); buffers are DeviceArray
     def uq(self, U_C, Q_C, Grid_P, det, FLOW, ROE, n_launch: int) -> None:
         _check(_lib.cfdx_uq(self.h, _ptr(U_C), _ptr(Q_C), _ptr(Grid_P), _ptr(det), _ptr(FLOW), _ptr(ROE), n_launch))
 
@@ -253,14 +283,29 @@ class Ctx:
                               _ptr(F), _ptr(E), _ptr(G), _ptr(FLOW), n_launch, C.byref(params)))
 
 
-def _to_device(a: np.ndarray, device):
-    import torch
-    t = torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1))
-    return t.to(device)
+class DeviceArray:
+    """hipMalloc'd buffer with the numpy dtype / shape of its host image."""
 
+    def __init__(self, ctx: Ctx, nbytes: int, dtype, shape):
+        p = _vp()
+        _check(_lib.cfdx_device_alloc(ctx.h, nbytes, C.byref(p)))
+        self.ctx, self.ptr, self.nbytes, self.dtype, self.shape = ctx, p.value, nbytes, np.dtype(dtype), tuple(shape)
 
-def _from_device(t, like: np.ndarray) -> np.ndarray:
-    return t.cpu().numpy().view(like.dtype).reshape(like.shape)
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(_lib.cfdx_copy_to_host(self.ctx.h, _ptr(out), _vp(self.ptr), self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ctx.h:
+            _check(_lib.cfdx_device_free(self.ctx.h, _vp(self.ptr)))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DropInSolver:
@@ -269,22 +314,18 @@ class DropInSolver:
     vectors U1 / U2 into the viscous kernel at j = 1, 2). Buffers stay in HBM between
     stages; periodic single subdomain (no halo exchange / boundary fill)."""
 
-    def __init__(self, case: dict, ctx: Ctx | None = None, device: str = "cuda"):
-        import torch
-        self.ctx = ctx or Ctx()
-        self.n = case["Grid_P"].shape[0]
+    def __init__(self, case: dict, ctx: Ctx | None = None):
+        self.ctx = c = ctx or Ctx()
+        self.n = n = case["Grid_P"].shape[0]
         self.case = case
-        dev = torch.device(device)
-        self.Grid_P = _to_device(case["Grid_P"], dev)
-        self.det = torch.from_numpy(case["det"]).to(dev)
-        self.metric = torch.from_numpy(case["metric"]).to(dev)
-        self.deter = torch.from_numpy(case["deter"]).to(dev)
-        self.FLOW = _to_device(case["FLOW"], dev)
-        z = lambda k: torch.zeros((self.n, k), dtype=torch.float64, device=dev)
-        self.U0, self.U1, self.U2, self.Q_C = z(5), z(5), z(5), z(5)
-        self.F, self.E, self.G, self.Fv, self.Ev, self.Gv = (z(5) for _ in range(6))
-        self.Uscratch = z(5)
-        self.ROE = z(ROE_N)
+        self.Grid_P = c.to_device(case["Grid_P"])
+        self.det = c.to_device(case["det"])
+        self.metric = c.to_device(case["metric"])
+        self.deter = c.to_device(case["deter"])
+        self.FLOW = c.to_device(case["FLOW"])
+        self.U0, self.U1, self.U2, self.Q_C, self.Uscratch = (c.zeros((n, 5)) for _ in range(5))
+        self.F, self.E, self.G, self.Fv, self.Ev, self.Gv = (c.zeros((n, 5)) for _ in range(6))
+        self.ROE = c.zeros((n, ROE_N))
 
     def step(self, nsteps: int, mach: float, dt: float, params: Params) -> None:
         n = self.n
@@ -299,8 +340,7 @@ class DropInSolver:
                        self.metric, self.ROE, self.Fv, self.Ev, self.Gv, self.F, self.E, self.G, self.FLOW, n, params)
 
     def flow(self) -> np.ndarray:
-        self.ctx.sync()
-        return _from_device(self.FLOW, self.case["FLOW"])
+        return self.FLOW.numpy()
 
 
 # ---------------------------------------------------------------------------

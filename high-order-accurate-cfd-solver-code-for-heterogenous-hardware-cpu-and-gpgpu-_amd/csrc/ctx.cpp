@@ -21,6 +21,13 @@ extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
         }
         c->owns_stream = true;
     }
+    for (auto &ev : c->ev) {
+        hipError_t e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+            cfdx_ctx_destroy(c);
+            return cfdx::hip_fail(e, "hipEventCreate");
+        }
+    }
     *out = c;
     return CFDX_OK;
 }
@@ -32,8 +39,10 @@ extern "C" int cfdx_ctx_destroy(cfdx_ctx *ctx)
     if (hipSetDevice(ctx->device) == hipSuccess && ctx->owns_stream) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = cfdx::hip_fail(e, "hipStreamSynchronize");
-        hipStreamDestroy(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
     }
+    for (auto &ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
     delete ctx;
     return rc;
 }
@@ -43,7 +52,7 @@ extern "C" int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream)
     CFDX_TRY(cfdx::enter(ctx));
     if (ctx->owns_stream) {
         CFDX_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        hipStreamDestroy(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
         ctx->owns_stream = false;
     }
     if (stream) {
@@ -61,5 +70,75 @@ extern "C" int cfdx_sync(cfdx_ctx *ctx)
 {
     CFDX_TRY(cfdx::enter(ctx));
     CFDX_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_device_count(int *count)
+{
+    if (!count) return cfdx::fail(CFDX_E_INVALID, "device_count: null");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return cfdx::hip_fail(e, "hipGetDeviceCount");
+    }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_device_alloc(cfdx_ctx *ctx, size_t bytes, void **dptr)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if (!dptr) return cfdx::fail(CFDX_E_INVALID, "device_alloc: null out");
+    *dptr = nullptr;
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return cfdx::fail(CFDX_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_device_free(cfdx_ctx *ctx, void *dptr)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if (!dptr) return CFDX_OK;
+    CFDX_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    CFDX_HIP(hipFree(dptr), "hipFree");
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_copy_to_device(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if ((!dst || !src) && bytes) return cfdx::fail(CFDX_E_INVALID, "copy_to_device: null pointer");
+    if (!bytes) return CFDX_OK;
+    CFDX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync H2D");
+    CFDX_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_copy_to_host(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if ((!dst || !src) && bytes) return cfdx::fail(CFDX_E_INVALID, "copy_to_host: null pointer");
+    if (!bytes) return CFDX_OK;
+    CFDX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
+    CFDX_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_event_record(cfdx_ctx *ctx, int slot)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if (slot < 0 || slot >= 8) return cfdx::fail(CFDX_E_INVALID, "event slot %d not in 0..7", slot);
+    CFDX_HIP(hipEventRecord(ctx->ev[slot], ctx->stream), "hipEventRecord");
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_event_elapsed(cfdx_ctx *ctx, int a, int b, double *ms)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if (a < 0 || a >= 8 || b < 0 || b >= 8 || !ms) return cfdx::fail(CFDX_E_INVALID, "event_elapsed: bad slot");
+    CFDX_HIP(hipEventSynchronize(ctx->ev[b]), "hipEventSynchronize");
+    float f = 0;
+    CFDX_HIP(hipEventElapsedTime(&f, ctx->ev[a], ctx->ev[b]), "hipEventElapsedTime");
+    *ms = f;
     return CFDX_OK;
 }

@@ -8,6 +8,7 @@ struct cfdx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
+    hipEvent_t ev[8] = {};
 };
 
 namespace cfdx {

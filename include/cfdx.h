@@ -95,6 +95,22 @@ int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream);
 void *cfdx_ctx_stream(cfdx_ctx *ctx);
 int cfdx_sync(cfdx_ctx *ctx);
 
+/* Number of HIP devices visible to this process. */
+int cfdx_device_count(int *count);
+
+/* Device memory and copies on the context's device (the drop-in kernels take
+ * device pointers; callers may equally use their own hipMalloc'd buffers).
+ * Copies are ordered on the context stream and complete before returning. */
+int cfdx_device_alloc(cfdx_ctx *ctx, size_t bytes, void **dptr);
+int cfdx_device_free(cfdx_ctx *ctx, void *dptr);
+int cfdx_copy_to_device(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int cfdx_copy_to_host(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* HIP events on the context stream: record into one of 8 slots, then the
+ * elapsed device time between two recorded slots (waits for the later one). */
+int cfdx_event_record(cfdx_ctx *ctx, int slot);
+int cfdx_event_elapsed(cfdx_ctx *ctx, int slot_a, int slot_b, double *ms);
+
 /* ------------------------------------------------------------------------ */
 /* 1. Drop-in kernels (device pointers, reference argument order)           */
 /* ------------------------------------------------------------------------ */

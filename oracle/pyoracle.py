@@ -125,5 +125,20 @@ class Oracle:
         return out
 
 
+    def rhs(self, case: dict, prm: Params) -> np.ndarray:
+        """Stage-0 right-hand side L (Weno.cl:3078-3082) of case['FLOW'], shape (n_nodes, 5):
+        Weno with del_t = 1 and U0 = 0 returns U1 = L exactly."""
+        n = case["Grid_P"].shape[0]
+        FLOW = case["FLOW"].copy()
+        w = self.new_work(n)
+        self.uq(n, w["U0"], w["Q_C"], case["Grid_P"], case["det"], FLOW, w["ROE"])
+        self.viscous(n, w["U0"], w["F"], w["E"], w["G"], w["Fv"], w["Ev"], w["Gv"], case["Grid_P"], case["metric"],
+                     case["det"], FLOW, prm)
+        zero = np.zeros_like(w["U0"])
+        self.weno(n, 0, 1.0, 1.0, case["deter"], case["det"], zero, w["U1"], w["U2"], case["Grid_P"], w["Q_C"],
+                  case["metric"], w["ROE"], w["Fv"], w["Ev"], w["Gv"], w["F"], w["E"], w["G"], FLOW, prm)
+        return w["U1"]
+
+
 def available(which: str) -> bool:
     return os.path.exists(LIB_ORACLE if which == "port" else LIB_REF)
