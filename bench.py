@@ -55,9 +55,9 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(cfdx, max_seconds: float = 20.0):
+def cpu_baseline(cfdx, max_seconds: float = 15.0):
     """The reference's own kernels (oracle/_ref, gcc -O3, OpenMP) or the restatement, on a
-    bounded sample of the same workload: a 48^3 block of the curvilinear TGV, whole RK3 steps."""
+    bounded sample of the same workload: a 64^3 block of the curvilinear TGV, whole RK3 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
@@ -67,7 +67,7 @@ def cpu_baseline(cfdx, max_seconds: float = 20.0):
     orc = pyoracle.Oracle(kind)
     cores = len(os.sched_getaffinity(0))
     orc.set_threads(cores)
-    n = 48
+    n = 64
     h = 2 * math.pi / n
     case = cfdx.periodic_case(n, n, n, grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * h,
                               phases=(0.3, 1.1, 2.0), init="tgv", mach=0.1)
@@ -78,7 +78,7 @@ def cpu_baseline(cfdx, max_seconds: float = 20.0):
     orc.steps(case, 1, 0.1, dt, prm, work)
     t1 = time.perf_counter()
     steps = max(1, int(max_seconds / max(t1 - t0, 1e-6)) - 1)
-    steps = min(steps, 50)
+    steps = min(steps, 100)
     t2 = time.perf_counter()
     orc.steps(case, steps, 0.1, dt, prm, work)
     t3 = time.perf_counter()
@@ -96,6 +96,7 @@ def main() -> int:
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
+    ap.add_argument("--copy-mib", type=int, default=256, help="size of the reference copy kernel run (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -142,6 +143,15 @@ def main() -> int:
     flow = block.get_flow()
     finite = bool(np.isfinite(cfdx_matrix(flow)).all())
 
+    # reference copy kernel of known size (scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)
+    nb = args.copy_mib << 20
+    a, b = ctx.zeros(nb // 8), ctx.zeros(nb // 8)
+    ctx.record(2)
+    ctx.copy(b, a)
+    ctx.record(3)
+    copy_ms = ctx.elapsed_ms(2, 3)
+    a.free(); b.free()
+
     line = {
         "metric": "cell-updates/s",
         "value": value,
@@ -163,6 +173,7 @@ def main() -> int:
         "hbm_alg_frac_whole_step": ALG_BYTES_PER_CELL_UPDATE * value / (HBM_PEAK_GBS * 1e9),
         "kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in tk.items() if k.endswith("_ms")},
         "state_finite": finite,
+        "copy_kernel": {"bytes_moved": 2 * nb, "ms": copy_ms, "GBs": 2 * nb / (copy_ms / 1e3) / 1e9},
     }
     if not args.no_cpu:
         cb = cpu_baseline(cfdx)

@@ -76,6 +76,7 @@ EXPORTS = {
     "cfdx_device_free": (C.c_int, [_vp, _vp]),
     "cfdx_copy_to_device": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "cfdx_copy_to_host": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "cfdx_device_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "cfdx_event_record": (C.c_int, [_vp, C.c_int]),
     "cfdx_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_d)]),
     "cfdx_uq": (C.c_int, [_vp] * 7 + [_i64]),
@@ -264,6 +265,9 @@ class Ctx:
         d = DeviceArray(self, a.nbytes, a.dtype, a.shape)
         _check(_lib.cfdx_copy_to_device(self.h, _vp(d.ptr), _ptr(a), a.nbytes))
         return d
+
+    def copy(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int | None = None) -> None:
+        _check(_lib.cfdx_device_copy(self.h, _vp(dst.ptr), _vp(src.ptr), nbytes or src.nbytes))
 
     def zeros(self, shape, dtype=np.float64) -> "DeviceArray":
         return self.to_device(np.zeros(shape, dtype=dtype))

@@ -23,14 +23,14 @@ __device__ __forceinline__ void prim_to_q(T rho, T u, T v, T w, T p, T (&q)[5])
     q[1] = rho * u;
     q[2] = rho * v;
     q[3] = rho * w;
-    q[4] = (p / T(0.4)) + ke;
+    q[4] = p * T(2.5) + ke;   // p / (gamma - 1)
 }
 
 // Total enthalpy H = (E + p) / rho of primitives (U_Q.cl:73-74).
 template <typename T>
 __device__ __forceinline__ T enthalpy(T rho, T u, T v, T w, T p)
 {
-    return ((p / T(0.4)) + T(0.5) * rho * (u * u + v * v + w * w) + p) / rho;
+    return (p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w)) / rho;   // (p/0.4 + ke + p) / rho
 }
 
 // Roe average of the face between states a and b (U_Q.cl:60-109): (u, v, w, h, rho).
@@ -52,13 +52,13 @@ template <typename T>
 __device__ __forceinline__ void node_flux(T rho, T u, T v, T w, T p, T mx, T my, T mz, T D, T (&f)[5])
 {
     const T V = mx * u + my * v + mz * w;
-    const T E = (p / T(0.4)) + T(0.5) * rho * (u * u + v * v + w * w);
+    const T Ep = p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w);   // E + p
     const T rV = rho * V;
     f[0] = D * rV;
     f[1] = D * (rV * u + mx * p);
     f[2] = D * (rV * v + my * p);
     f[3] = D * (rV * w + mz * p);
-    f[4] = D * ((E + p) * V);
+    f[4] = D * (Ep * V);
 }
 
 // WCNS-E5 interpolation of one characteristic field to the face between W[2] and W[3]
@@ -93,19 +93,25 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     const T ISm0 = c13 * s2 * s2 + c14 * u0 * u0;
     const T ISm1 = c13 * s1 * s1 + c14 * u1 * u1;
     const T ISm2 = c13 * s0 * s0 + c14 * u2 * u2;
+    // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2; one reciprocal serves both sides
+    T np_, dp_, nn_, dn_;
     {
         T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
         a0 *= a0; a1 *= a1; a2 *= a2;
-        // w_k proportional to d_k * prod_{m != k} a_m
         const T w0 = T(1.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(5.0 / 16.0) * (a0 * a1);
-        pos = (w0 * h_r + w1 * h_c + w2 * h_l) / (w0 + w1 + w2);
+        np_ = w0 * h_r + w1 * h_c + w2 * h_l;
+        dp_ = w0 + w1 + w2;
     }
     {
         T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
         a0 *= a0; a1 *= a1; a2 *= a2;
         const T w0 = T(5.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(1.0 / 16.0) * (a0 * a1);
-        neg = (w0 * h_c + w1 * h_l + w2 * h_ll) / (w0 + w1 + w2);
+        nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
+        dn_ = w0 + w1 + w2;
     }
+    const T r = T(1) / (dp_ * dn_);
+    pos = np_ * (dn_ * r);
+    neg = nn_ * (dp_ * r);
 }
 
 // Characteristic reconstruction of the two interface states of one face
@@ -133,8 +139,8 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
     const T g4aa = T(0.4) * iaa;
     const T one_m = T(1) - phi * iaa;
     const T cv = kz * v - ky * w, cw = kx * w - kz * u, cu = ky * u - kx * v;  // n x u components
-    const T phi_g = phi / T(0.4);
-    const T hh = (phi + a * a) / T(0.4);
+    const T phi_g = phi * T(2.5);
+    const T hh = (phi + a * a) * T(2.5);
 
 #pragma unroll
     for (int m = 0; m < 5; ++m) { qp[m] = T(0); qn[m] = T(0); }
@@ -188,12 +194,10 @@ __device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k
     lam = Un + sqrt_(T(1.4) * P * r * kk2);
 }
 
-// Face flux: LLF with alpha = max(|lambda_5|) (Weno.cl:2859-2947) plus the explicit
-// midpoint correction - delta2 F / 24 + 7 delta4 F / 5760 from node fluxes nf at offsets
-// -2..+3 (Weno.cl:2951-3058). Output is F-hat of the face (det-scaled, contravariant).
-template <typename T, typename TO>
-__device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
-                                          const T (&nf)[6][5], TO (&out)[5])
+// LLF face flux with alpha = max(|lambda_5|) of the two interface states (Weno.cl:2859-2947).
+template <typename T>
+__device__ __forceinline__ void face_llf(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
+                                         T (&llf)[5])
 {
     T qp[5], qn[5];
     characteristic_states(q, roe, k0, k1, k2, eps, qp, qn);
@@ -204,12 +208,29 @@ __device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5],
     const T al = fabs_(lp) >= fabs_(ln) ? fabs_(lp) : fabs_(ln);
     const T ad = al * detf;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        const T llf = T(0.5) * (cp[c] + cn[c] - ad * (qp[c] - qn[c]));
-        const T d2 = T(1.0 / 48.0) * (T(-5) * nf[0][c] + T(39) * nf[1][c] - T(34) * nf[2][c] - T(34) * nf[3][c] + T(39) * nf[4][c] - T(5) * nf[5][c]);
-        const T d4 = T(0.5) * (nf[0][c] - T(3) * nf[1][c] + T(2) * nf[2][c] + T(2) * nf[3][c] - T(3) * nf[4][c] + nf[5][c]);
-        out[c] = TO(llf - T(1.0 / 24.0) * d2 + T(7.0 / 5760.0) * d4);
-    }
+    for (int c = 0; c < 5; ++c) llf[c] = T(0.5) * (cp[c] + cn[c] - ad * (qp[c] - qn[c]));
+}
+
+// Explicit midpoint correction - delta2 F / 24 + 7 delta4 F / 5760 of one component from the node
+// fluxes at offsets -2..+3 (Weno.cl:2951-3058).
+template <typename T>
+__device__ __forceinline__ T midpoint_correction(T f0, T f1, T f2, T f3, T f4, T f5)
+{
+    const T d2 = T(1.0 / 48.0) * (T(-5) * f0 + T(39) * f1 - T(34) * f2 - T(34) * f3 + T(39) * f4 - T(5) * f5);
+    const T d4 = T(0.5) * (f0 - T(3) * f1 + T(2) * f2 + T(2) * f3 - T(3) * f4 + f5);
+    return -T(1.0 / 24.0) * d2 + T(7.0 / 5760.0) * d4;
+}
+
+// Face flux F-hat = LLF + midpoint correction from node fluxes nf at offsets -2..+3.
+template <typename T, typename TO>
+__device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
+                                          const T (&nf)[6][5], TO (&out)[5])
+{
+    T llf[5];
+    face_llf(q, roe, k0, k1, k2, detf, eps, llf);
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        out[c] = TO(llf[c] + midpoint_correction(nf[0][c], nf[1][c], nf[2][c], nf[3][c], nf[4][c], nf[5][c]));
 }
 
 // Viscous + MTS-SGS node fluxes (Viscous.cl:439-631) of one node.
@@ -218,7 +239,30 @@ __device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5],
 //   m[9]     = node metrics zeta_x..xi_z; D = det
 //   KS[3]    = MTS scale-similarity terms KS_x, KS_y, KS_z (Viscous.cl:493-495)
 // Writes the contravariant det-scaled viscous fluxes fvis[0..2] = Fv, Ev, Gv and returns mu.
-struct ViscParams { double mach, reyl, pr, free_t, c_mts, c_t; };
+struct ViscParams {
+    double mach, reyl, pr, free_t, c_mts, c_t;
+    // derived on the host (uniform per launch)
+    double inv_reyl;     // 1 / Re
+    double kheat;        // 1.4 / (Pr Re)
+    double suth;         // 110.4 / T_inf
+    double suth1;        // 1 + 110.4 / T_inf
+    double tfac;         // 1.4 M^2 0.4
+    double kQ;           // -1 / (1.4 0.4 M^2)
+    double inv_ct;       // 1 / C_T
+};
+
+inline ViscParams make_visc_params(double mach, double reyl, double pr, double free_t, double c_mts, double c_t)
+{
+    ViscParams v{mach, reyl, pr, free_t, c_mts, c_t, 0, 0, 0, 0, 0, 0, 0};
+    v.inv_reyl = 1.0 / reyl;
+    v.kheat = 1.4 / (pr * reyl);
+    v.suth = 110.4 / free_t;
+    v.suth1 = 1.0 + v.suth;
+    v.tfac = 1.4 * mach * mach * 0.4;
+    v.kQ = -(1.0 / (1.4 * 0.4 * mach * mach));
+    v.inv_ct = 1.0 / c_t;
+    return v;
+}
 
 template <typename T> __device__ __forceinline__ T cbrt_(T x);
 template <> __device__ __forceinline__ double cbrt_<double>(double x) { return cbrt(x); }
@@ -228,8 +272,7 @@ template <typename T>
 __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], T u, T v, T w, const T (&m)[9], T D,
                                           const T (&KS)[3], const ViscParams &vp, T (&fvis)[3][5])
 {
-    const T Mach = T(vp.mach), Reyl = T(vp.reyl), Pr = T(vp.pr), Free_t = T(vp.free_t);
-    const T C_MTS = T(vp.c_mts), C_T = T(vp.c_t);
+    const T C_MTS = T(vp.c_mts);
     const T iU0 = T(1) / U[0];
     const T iU02 = iU0 * iU0;
     T ug[3], vg[3], wg[3], tg[3];
@@ -257,14 +300,13 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     const T ez = tg[0] * m[2] + tg[1] * m[5] + tg[2] * m[8];
 
     // Viscous.cl:455-458: T = gamma M^2 (gamma-1) e, Sutherland mu with S = 110.4
-    const T d_temp = T(1.4) * Mach * Mach * T(0.4) * (U[4] * iU0 - T(0.5) * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * iU02));
+    const T d_temp = T(vp.tfac) * (U[4] * iU0 - T(0.5) * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * iU02));
     const T st = sqrt_(d_temp);
-    const T S = T(110.4) / Free_t;
-    const T mu = (st * st * st) * ((T(1) + S) / (d_temp + S));
+    const T mu = (st * st * st) * (T(vp.suth1) / (d_temp + T(vp.suth)));
     // Viscous.cl:461-476 heat flux and Stokes stresses
-    const T kq = -(mu * T(1.4) / (Pr * Reyl));
+    const T kq = -(mu * T(vp.kheat));
     const T qx = kq * ex, qy = kq * ey, qz = kq * ez;
-    const T mr = mu / Reyl;
+    const T mr = mu * T(vp.inv_reyl);
     const T txx = mr * (T(4.0 / 3.0) * ux - T(2.0 / 3.0) * vy - T(2.0 / 3.0) * wz);
     const T tyy = mr * (T(4.0 / 3.0) * vy - T(2.0 / 3.0) * ux - T(2.0 / 3.0) * wz);
     const T tzz = mr * (T(4.0 / 3.0) * wz - T(2.0 / 3.0) * ux - T(2.0 / 3.0) * vy);
@@ -281,8 +323,9 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     for (int d = 0; d < 3; ++d) {
         const T sk = sqrt_(KS[d]);
         if (mod != T(0)) {
-            const T R = (DEL * mod) / sk;
-            eddy[d] = (C_MTS / (T(1) + R / C_T)) * DEL * sk;
+            // C_MTS / (1 + R / C_T) * DEL * sk with R = DEL |S| / sk, written with one division
+            // (sk = 0 gives R = inf and a zero eddy viscosity either way)
+            eddy[d] = (C_MTS * DEL * sk * sk) / (sk + DEL * mod * T(vp.inv_ct));
         } else {
             eddy[d] = C_MTS * DEL * sk;
         }
@@ -291,7 +334,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     const T m2U = T(-2) * U[0];
     const T Sxy = m2U * eddy[0] * sxy, Syz = m2U * eddy[1] * syz, Sxz = m2U * eddy[2] * sxz;
     const T Sxx = m2U * eddy[0] * sxx, Syy = m2U * eddy[1] * syy, Szz = m2U * eddy[2] * szz;
-    const T kQ = -(T(1) / (T(1.4) * T(0.4) * Mach * Mach));
+    const T kQ = T(vp.kQ);
     const T Qx = kQ * eddy[0] * ex, Qy = kQ * eddy[1] * ey, Qz = kQ * eddy[2] * ez;
     const T DX = (Sxx + Sxy + Sxz) * u, DY = (Sxy + Syy + Syz) * v, DZ = (Sxz + Syz + Szz) * w;
     // H = Q + DEL and D = 0.4 Q (Viscous.cl:549-557)

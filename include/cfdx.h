@@ -106,6 +106,10 @@ int cfdx_device_free(cfdx_ctx *ctx, void *dptr);
 int cfdx_copy_to_device(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int cfdx_copy_to_host(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes);
 
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and stores);
+ * bytes must be a multiple of 16. Also the reference point for HBM counters. */
+int cfdx_device_copy(cfdx_ctx *ctx, void *dst, const void *src, size_t bytes);
+
 /* HIP events on the context stream: record into one of 8 slots, then the
  * elapsed device time between two recorded slots (waits for the later one). */
 int cfdx_event_record(cfdx_ctx *ctx, int slot);

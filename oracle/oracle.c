@@ -288,8 +288,7 @@ typedef struct {
  * the five fields at :1280-2187), :2188-2287 (back projection), :2291-2507
  * (Euler fluxes), :2511-2581 (eigenvalues), :2585-2727 (contravariant and det).
  */
-static void weno_face(const orc_vec5 *s[6], const double roe[6], const double k[3], double detf, double Epsilon,
-                      orc_face *out)
+static void face_eigen(const double roe[6], const double k[3], double R[5][5], double L[5][5])
 {
     const double ru = roe[0], rv = roe[1], rw = roe[2], rh = roe[4], rr = roe[5];
     const double kk = sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
@@ -301,7 +300,6 @@ static void weno_face(const orc_vec5 *s[6], const double roe[6], const double k[
     const double beta = 1.0 / (sqrt(2.0) * rr * roe_a);
     const double aa = roe_a * roe_a;
 
-    double R[5][5], L[5][5];
     R[0][0] = kx; R[0][1] = ky; R[0][2] = kz; R[0][3] = alpha; R[0][4] = alpha;
     R[1][0] = kx * ru; R[1][1] = ky * ru - kz * rr; R[1][2] = kz * ru + ky * rr;
     R[1][3] = alpha * (ru + kx * roe_a); R[1][4] = alpha * (ru - kx * roe_a);
@@ -340,6 +338,54 @@ static void weno_face(const orc_vec5 *s[6], const double roe[6], const double k[
     L[4][2] = -beta * (0.4 * rv + ky * roe_a);
     L[4][3] = -beta * (0.4 * rw + kz * roe_a);
     L[4][4] = beta * 0.4;
+}
+
+/* WCNS interpolation of one characteristic field, both sides (Weno.cl:1056-1279) */
+static void wcns_pair(const double Wc[6], double Epsilon, double *pos, double *neg)
+{
+    const double m2 = Wc[0], m1 = Wc[1], z0 = Wc[2], p1 = Wc[3], p2 = Wc[4], p3 = Wc[5];
+    double hp[3], hm[3], ISp[3], ISm[3], wp[3], wm[3], Wp[3], Wm[3];
+    hp[0] = (15.0 / 8.0) * p1 - (5.0 / 4.0) * p2 + (3.0 / 8.0) * p3;
+    hp[1] = (3.0 / 8.0) * z0 + (3.0 / 4.0) * p1 - (1.0 / 8.0) * p2;
+    hp[2] = (-1.0 / 8.0) * m1 + (3.0 / 4.0) * z0 + (3.0 / 8.0) * p1;
+    hm[0] = (3.0 / 8.0) * z0 + (3.0 / 4.0) * p1 - (1.0 / 8.0) * p2;
+    hm[1] = (-1.0 / 8.0) * m1 + (3.0 / 4.0) * z0 + (3.0 / 8.0) * p1;
+    hm[2] = (3.0 / 8.0) * m2 - (5.0 / 4.0) * m1 + (15.0 / 8.0) * z0;
+    ISm[2] = (13.0 / 12.0) * (pow(m2 - 2.0 * m1 + z0, 2)) + (1.0 / 4.0) * (pow(m2 - 4.0 * m1 + 3.0 * z0, 2));
+    ISm[1] = (13.0 / 12.0) * (pow(m1 - 2.0 * z0 + p1, 2)) + (1.0 / 4.0) * (pow(m1 - p1, 2));
+    ISm[0] = (13.0 / 12.0) * (pow(z0 - 2.0 * p1 + p2, 2)) + (1.0 / 4.0) * (pow(3.0 * z0 - 4.0 * p1 + p2, 2));
+    ISp[2] = (13.0 / 12.0) * (pow(m1 - 2.0 * z0 + p1, 2)) + (1.0 / 4.0) * (pow(m1 - 4.0 * z0 + 3.0 * p1, 2));
+    ISp[1] = (13.0 / 12.0) * (pow(z0 - 2.0 * p1 + p2, 2)) + (1.0 / 4.0) * (pow(z0 - p2, 2));
+    ISp[0] = (13.0 / 12.0) * (pow(p1 - 2.0 * p2 + p3, 2)) + (1.0 / 4.0) * (pow(3.0 * p1 - 4.0 * p2 + p3, 2));
+    wp[0] = (1.0 / 16.0) / (pow(Epsilon + (ISp[0]), 2.0));
+    wp[1] = (5.0 / 8.0) / (pow(Epsilon + (ISp[1]), 2.0));
+    wp[2] = (5.0 / 16.0) / (pow(Epsilon + (ISp[2]), 2.0));
+    wm[0] = (5.0 / 16.0) / (pow(Epsilon + (ISm[0]), 2.0));
+    wm[1] = (5.0 / 8.0) / (pow(Epsilon + (ISm[1]), 2.0));
+    wm[2] = (1.0 / 16.0) / (pow(Epsilon + (ISm[2]), 2.0));
+    for (int q = 0; q < 3; ++q) {
+        Wp[q] = wp[q] / (wp[0] + wp[1] + wp[2]);
+        Wm[q] = wm[q] / (wm[0] + wm[1] + wm[2]);
+    }
+    *pos = Wp[0] * hp[0] + Wp[1] * hp[1] + Wp[2] * hp[2];
+    *neg = Wm[0] * hm[0] + Wm[1] * hm[1] + Wm[2] * hm[2];
+}
+
+void orc_wcns_pair(const double W[6], double eps, double *pos, double *neg) { wcns_pair(W, eps, pos, neg); }
+
+void orc_face_eigen(const double roe[6], const double k[3], double L[25], double R[25])
+{
+    double l[5][5], r[5][5];
+    face_eigen(roe, k, r, l);
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) { L[5 * i + j] = l[i][j]; R[5 * i + j] = r[i][j]; }
+}
+
+static void weno_face(const orc_vec5 *s[6], const double roe[6], const double k[3], double detf, double Epsilon,
+                      orc_face *out)
+{
+    double R[5][5], L[5][5];
+    face_eigen(roe, k, R, L);
 
     /* W[m][c] = characteristic field c at stencil point m (Weno.cl:415-455) */
     double W[6][5];
@@ -349,33 +395,8 @@ static void weno_face(const orc_vec5 *s[6], const double roe[6], const double k[
 
     double cpos[5], cneg[5];
     for (int c = 0; c < 5; ++c) {
-        /* offsets: m1 = W[1] (-1), z0 = W[2] (0), p1 = W[3], p2 = W[4], p3 = W[5], m2 = W[0] */
-        const double m2 = W[0][c], m1 = W[1][c], z0 = W[2][c], p1 = W[3][c], p2 = W[4][c], p3 = W[5][c];
-        double hp[3], hm[3], ISp[3], ISm[3], wp[3], wm[3], Wp[3], Wm[3];
-        hp[0] = (15.0 / 8.0) * p1 - (5.0 / 4.0) * p2 + (3.0 / 8.0) * p3;
-        hp[1] = (3.0 / 8.0) * z0 + (3.0 / 4.0) * p1 - (1.0 / 8.0) * p2;
-        hp[2] = (-1.0 / 8.0) * m1 + (3.0 / 4.0) * z0 + (3.0 / 8.0) * p1;
-        hm[0] = (3.0 / 8.0) * z0 + (3.0 / 4.0) * p1 - (1.0 / 8.0) * p2;
-        hm[1] = (-1.0 / 8.0) * m1 + (3.0 / 4.0) * z0 + (3.0 / 8.0) * p1;
-        hm[2] = (3.0 / 8.0) * m2 - (5.0 / 4.0) * m1 + (15.0 / 8.0) * z0;
-        ISm[2] = (13.0 / 12.0) * (pow(m2 - 2.0 * m1 + z0, 2)) + (1.0 / 4.0) * (pow(m2 - 4.0 * m1 + 3.0 * z0, 2));
-        ISm[1] = (13.0 / 12.0) * (pow(m1 - 2.0 * z0 + p1, 2)) + (1.0 / 4.0) * (pow(m1 - p1, 2));
-        ISm[0] = (13.0 / 12.0) * (pow(z0 - 2.0 * p1 + p2, 2)) + (1.0 / 4.0) * (pow(3.0 * z0 - 4.0 * p1 + p2, 2));
-        ISp[2] = (13.0 / 12.0) * (pow(m1 - 2.0 * z0 + p1, 2)) + (1.0 / 4.0) * (pow(m1 - 4.0 * z0 + 3.0 * p1, 2));
-        ISp[1] = (13.0 / 12.0) * (pow(z0 - 2.0 * p1 + p2, 2)) + (1.0 / 4.0) * (pow(z0 - p2, 2));
-        ISp[0] = (13.0 / 12.0) * (pow(p1 - 2.0 * p2 + p3, 2)) + (1.0 / 4.0) * (pow(3.0 * p1 - 4.0 * p2 + p3, 2));
-        wp[0] = (1.0 / 16.0) / (pow(Epsilon + (ISp[0]), 2.0));
-        wp[1] = (5.0 / 8.0) / (pow(Epsilon + (ISp[1]), 2.0));
-        wp[2] = (5.0 / 16.0) / (pow(Epsilon + (ISp[2]), 2.0));
-        wm[0] = (5.0 / 16.0) / (pow(Epsilon + (ISm[0]), 2.0));
-        wm[1] = (5.0 / 8.0) / (pow(Epsilon + (ISm[1]), 2.0));
-        wm[2] = (1.0 / 16.0) / (pow(Epsilon + (ISm[2]), 2.0));
-        for (int q = 0; q < 3; ++q) {
-            Wp[q] = wp[q] / (wp[0] + wp[1] + wp[2]);
-            Wm[q] = wm[q] / (wm[0] + wm[1] + wm[2]);
-        }
-        cpos[c] = Wp[0] * hp[0] + Wp[1] * hp[1] + Wp[2] * hp[2];
-        cneg[c] = Wm[0] * hm[0] + Wm[1] * hm[1] + Wm[2] * hm[2];
+        const double Wc[6] = {W[0][c], W[1][c], W[2][c], W[3][c], W[4][c], W[5][c]};
+        wcns_pair(Wc, Epsilon, &cpos[c], &cneg[c]);
     }
 
     double qp[5], qn[5];

@@ -69,6 +69,13 @@ void orc_solver_steps(int64_t n_g, int64_t n_sd, int nsteps, double Mach, double
                       const orc_grid *Grid_P, const double *det, const orc_metric *metric,
                       const orc_deter *deter, orc_flow *FLOW, orc_work *w, const orc_params *prm);
 
+/* Unit-test hooks into the restatement's building blocks (liboracle.so only):
+ * the WCNS pair of one characteristic field (W[0..5] at offsets -2..+3 from the
+ * face's left node; Weno.cl:1056-1279), and the L / R eigenvector matrices of a
+ * face (row-major 5x5; Weno.cl:336-412). */
+void orc_wcns_pair(const double W[6], double eps, double *pos, double *neg);
+void orc_face_eigen(const double roe[6], const double k[3], double L[25], double R[25]);
+
 /* Identifies which implementation is loaded: 0 = restatement, 1 = reference kernels. */
 int orc_kind(void);
 /* Set the OpenMP thread count used by the loops above (0 = library default). */

@@ -33,16 +33,19 @@ def _loc_code(on):
     return 7 + (hash(tuple(on)) % 8)            # corners 7..14
 
 
-def wall_box(ni, nj, nk, mach=0.3, noise=0.05, seed=7):
-    """ni x nj x nk boundary+interior nodes, 3 ghost layers around them."""
+def wall_box(ni, nj, nk, mach=0.3, noise=0.05, seed=7, walls=(True, True, True)):
+    """ni x nj x nk boundary+interior nodes; along each walled direction a boundary layer
+    (loc 1..26 on the first / last plane) and 3 ghost layers (loc 100) beyond it; the other
+    directions wrap periodically."""
     H = 3
-    Ni, Nj, Nk = ni + 2 * H, nj + 2 * H, nk + 2 * H
-    lat = np.zeros((Ni, Nj, Nk), dtype=np.int64)
+    ext = [H if w else 0 for w in walls]
+    dims = (ni, nj, nk)
+    Ni, Nj, Nk = (dims[d] + 2 * ext[d] for d in range(3))
     order_core, order_ghost = [], []
     for k in range(Nk):
         for j in range(Nj):
             for i in range(Ni):
-                core = H <= i < H + ni and H <= j < H + nj and H <= k < H + nk
+                core = all(ext[d] <= (i, j, k)[d] < ext[d] + dims[d] for d in range(3))
                 (order_core if core else order_ghost).append((i, j, k))
     ids = {}
     for t, ijk in enumerate(order_core + order_ghost):
@@ -54,32 +57,44 @@ def wall_box(ni, nj, nk, mach=0.3, noise=0.05, seed=7):
     rng = np.random.default_rng(seed)
     ph = rng.uniform(0, 2 * np.pi, 3)
     a = 0.3 * hgrid
+    per = [2 * np.pi / hgrid / dims[d] if not walls[d] else None for d in range(3)]
 
     def X(s1, s2, s3):
         return (s1 + a * np.sin(s2 + ph[0]) * np.sin(s3 + ph[0]), s2 + a * np.sin(s3 + ph[1]) * np.sin(s1 + ph[1]),
                 s3 + a * np.sin(s1 + ph[2]) * np.sin(s2 + ph[2]))
 
+    def nb(i, j, k, d, o):
+        q = [i, j, k]
+        q[d] += o
+        if not walls[d]:
+            q[d] %= dims[d]
+        return ids.get(tuple(q), 0)
+
     jac = np.zeros((n + 1, 9))
     for (i, j, k), g in ids.items():
-        s = np.array([i, j, k], dtype=float) * hgrid
+        # periodic directions use a map period equal to the block length (2 pi / h nodes * h)
+        s = np.array([i, j, k], dtype=float) * np.array([2 * np.pi / dims[d] if not walls[d] else hgrid for d in range(3)])
         xyz[g] = X(*s)
-        # analytic Jacobian d(x,y,z)/d(i,j,k)
-        J = np.zeros((3, 3))
+        hs = np.array([2 * np.pi / dims[d] if not walls[d] else hgrid for d in range(3)])
         s1, s2, s3 = s
+        J = np.zeros((3, 3))
         J[0] = [1, a * np.cos(s2 + ph[0]) * np.sin(s3 + ph[0]), a * np.sin(s2 + ph[0]) * np.cos(s3 + ph[0])]
         J[1] = [a * np.sin(s3 + ph[1]) * np.cos(s1 + ph[1]), 1, a * np.cos(s3 + ph[1]) * np.sin(s1 + ph[1])]
         J[2] = [a * np.cos(s1 + ph[2]) * np.sin(s2 + ph[2]), a * np.sin(s1 + ph[2]) * np.cos(s2 + ph[2]), 1]
-        jac[g] = (J * hgrid).reshape(-1)
-        nb = lambda di, dj, dk: ids.get((i + di, j + dj, k + dk), 0)
-        G[g]["N0"], G[g]["N1"], G[g]["N2"] = nb(0, 1, 0), nb(1, 0, 0), nb(0, -1, 0)
-        G[g]["N3"], G[g]["N4"], G[g]["N5"] = nb(-1, 0, 0), nb(0, 0, 1), nb(0, 0, -1)
-        core = H <= i < H + ni and H <= j < H + nj and H <= k < H + nk
+        jac[g] = (J * hs[None, :]).reshape(-1)
+        G[g]["N0"], G[g]["N1"], G[g]["N2"] = nb(i, j, k, 1, 1), nb(i, j, k, 0, 1), nb(i, j, k, 1, -1)
+        G[g]["N3"], G[g]["N4"], G[g]["N5"] = nb(i, j, k, 0, -1), nb(i, j, k, 2, 1), nb(i, j, k, 2, -1)
+        core = all(ext[d] <= (i, j, k)[d] < ext[d] + dims[d] for d in range(3))
         if core:
-            on = [i == H, i == H + ni - 1, j == H, j == H + nj - 1, k == H, k == H + nk - 1]
+            on = []
+            for d in range(3):
+                c = (i, j, k)[d]
+                on += [walls[d] and c == ext[d], walls[d] and c == ext[d] + dims[d] - 1]
             G[g]["loc"] = _loc_code(on)
         else:
             G[g]["loc"] = 100
         G[g]["x"], G[g]["y"], G[g]["z"] = xyz[g]
+    del per
     jac[0] = np.eye(3).reshape(-1)
     det = np.ones(n + 1)
     metric = np.zeros((n + 1, 63))
@@ -106,27 +121,59 @@ def wall_box(ni, nj, nk, mach=0.3, noise=0.05, seed=7):
     return dict(Grid_P=G, det=det, metric=metric, deter=deter, FLOW=flow, n_core=n_core, n=n)
 
 
+def _branch_offsets(LI, LM2, LM1, LP1, LP2, weno_form):
+    """Stencil offsets of the loc branch Viscous.cl:285-435 / Weno.cl:2732-2856 selects
+    (later matches overwrite earlier ones), or None when no branch matches."""
+    used = None
+    if 0 < LI <= 52 and LP1 == 100:
+        used = (0, -1, -2, -3)
+    if 0 < LI <= 52 and LM1 == 100:
+        used = (0, 1, 2, 3)
+    if LI == 0 and 0 < LP1 <= 52 and LP2 == 100:
+        used = (1, 0, -1, -2)
+    if LI == 0 and LM1 > 0 and (LM1 <= 52 or not weno_form) and LM2 == 100:
+        used = (-1, 0, 1, 2)
+    if 100 not in (LM2, LM1, LP1, LP2) and (LI == 0 or ((not weno_form) and LI > 0)):
+        used = (1, -1, 2, -2)
+    return used
+
+
 def defined_mask(case, weno_form):
-    """Nodes where every direction takes one of the loc branches (value defined in the reference)."""
+    """Nodes whose value the reference defines: every direction takes one of the loc branches
+    and, for the Weno form, every node that branch reads has a defined viscous flux."""
     G = case["Grid_P"]
     loc = G["loc"]
     n1 = len(G)
-    ok = np.zeros(n1, dtype=bool)
     plus = {0: "N1", 1: "N0", 2: "N4"}
     minus = {0: "N3", 1: "N2", 2: "N5"}
-    for g in range(1, n1):
-        good = True
+
+    def walk(g, d):
+        idx = {0: g}
+        p = m = g
+        for s in (1, 2, 3):
+            p, m = G[p][plus[d]], G[m][minus[d]]
+            idx[s], idx[-s] = p, m
+        return idx
+
+    def own(g, wf):
+        out = []
         for d in range(3):
-            p1 = G[g][plus[d]]; p2 = G[p1][plus[d]]
-            m1 = G[g][minus[d]]; m2 = G[m1][minus[d]]
-            LI, LP1, LP2, LM1, LM2 = loc[g], loc[p1], loc[p2], loc[m1], loc[m2]
-            hit = (0 < LI <= 52 and LP1 == 100) or (0 < LI <= 52 and LM1 == 100) or \
-                  (LI == 0 and 0 < LP1 <= 52 and LP2 == 100) or \
-                  (LI == 0 and LM1 > 0 and (LM1 <= 52 or not weno_form) and LM2 == 100) or \
-                  (LI == 0 and 100 not in (LM2, LM1, LP1, LP2)) or \
-                  ((not weno_form) and LI > 0 and 100 not in (LM2, LM1, LP1, LP2))
-            good &= bool(hit)
-        ok[g] = good
+            idx = walk(g, d)
+            used = _branch_offsets(loc[g], loc[idx[-2]], loc[idx[-1]], loc[idx[1]], loc[idx[2]], wf)
+            if used is None:
+                return None
+            out.append([idx[o] for o in used])
+        return out
+
+    visc = np.zeros(n1, dtype=bool)
+    for g in range(1, n1):
+        visc[g] = own(g, False) is not None
+    if not weno_form:
+        return visc
+    ok = np.zeros(n1, dtype=bool)
+    for g in range(1, n1):
+        nodes = own(g, True)
+        ok[g] = nodes is not None and all(visc[x] for row in nodes for x in row)
     return ok
 
 

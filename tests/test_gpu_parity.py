@@ -69,10 +69,11 @@ def test_dropin_stage_periodic(ctx, orc):
     _compare(a, b, slice(1, None))
 
 
-def test_dropin_stage_wall_box(ctx, orc):
+@pytest.mark.parametrize("walls", [(True, True, True), (False, False, True)])
+def test_dropin_stage_walls(ctx, orc, walls):
     """Ghost layers + loc codes: one-sided stencils; compared where the reference defines the value."""
     import pyoracle
-    case = cases.wall_box(7, 8, 9)
+    case = cases.wall_box(7, 8, 9, walls=walls)
     prm_o = pyoracle.reference_params()
     a = _dropin_stage(ctx, case, 0.3, 0.01, cfdx.default_params())
     b = orc.stage_outputs(case, 0.3, 0.01, prm_o)
