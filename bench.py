@@ -95,6 +95,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
     ap.add_argument("--copy-mib", type=int, default=256, help="size of the reference copy kernel run (HBM counter scale)")
     args = ap.parse_args()
@@ -112,7 +113,8 @@ def main() -> int:
     mach = cdef["kw"]["mach"]
     dt = cfdx.cfl_dt(case, 0.5)
     n = case["n"]
-    block = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"])
+    precision = cfdx.MIXED if args.precision == "mixed" else cfdx.FP64
+    block = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision)
     block.set_case(case)
     del case
     print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
@@ -163,9 +165,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if precision == cfdx.FP64 else "f32 fluxes / f64 state",
         "data": "synthetic",
-        "config": dict(spec, parallelism="single GPU, one periodic block", precision="fp64"),
+        "config": dict(spec, parallelism="single GPU, one periodic block", precision=args.precision),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_face (one of 3 per stage)", "kernel_ms": face_ms,

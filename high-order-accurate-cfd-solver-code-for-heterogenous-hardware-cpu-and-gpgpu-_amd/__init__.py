@@ -88,7 +88,15 @@ EXPORTS = {
     "cfdx_block_set_geometry": (C.c_int, [_vp, _vp, _vp, _vp]),
     "cfdx_block_set_flow": (C.c_int, [_vp, _vp]),
     "cfdx_block_get_flow": (C.c_int, [_vp, _vp]),
+    "cfdx_block_configure": (C.c_int, [_vp, _d, _d, C.POINTER(Params)]),
     "cfdx_block_step": (C.c_int, [_vp, _d, _d, C.c_int, C.POINTER(Params)]),
+    "cfdx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "cfdx_batch_destroy": (C.c_int, [_vp]),
+    "cfdx_batch_add": (C.c_int, [_vp, _vp]),
+    "cfdx_batch_remove": (C.c_int, [_vp, _vp]),
+    "cfdx_batch_size": (C.c_int, [_vp]),
+    "cfdx_batch_set_grid": (C.c_int, [_vp, C.c_uint32]),
+    "cfdx_batch_step": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cfdx_block_rhs": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "cfdx_block_set_timing": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_timing": (C.c_int, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(C.c_int)]),
@@ -355,6 +363,7 @@ class Block:
 
     def __init__(self, ni: int, nj: int, nk: int, precision: int = FP64, ctx: Ctx | None = None):
         self.ctx = ctx or Ctx()
+        self.precision = precision
         self.desc = BlockDesc(ni, nj, nk, 7, precision)
         h = _vp()
         _check(_lib.cfdx_block_create(self.ctx.h, C.byref(self.desc), C.byref(h)))
@@ -395,6 +404,9 @@ class Block:
         _check(_lib.cfdx_block_get_flow(self.h, _ptr(out)))
         return out
 
+    def configure(self, mach: float, dt: float, params: Params) -> None:
+        _check(_lib.cfdx_block_configure(self.h, mach, dt, C.byref(params)))
+
     def step(self, nsteps: int, mach: float, dt: float, params: Params) -> None:
         _check(_lib.cfdx_block_step(self.h, mach, dt, nsteps, C.byref(params)))
 
@@ -410,6 +422,45 @@ class Block:
         a, b, c, n = _d(), _d(), _d(), C.c_int()
         _check(_lib.cfdx_block_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(n)))
         return dict(visc_ms=a.value, face_ms=b.value, update_ms=c.value, launches=n.value)
+
+
+class Batch:
+    """Blocks of different sizes / run parameters stepped together (one launch per kernel class)."""
+
+    def __init__(self, precision: int = FP64, ctx: Ctx | None = None):
+        self.ctx = ctx or Ctx()
+        h = _vp()
+        _check(_lib.cfdx_batch_create(self.ctx.h, precision, C.byref(h)))
+        self.h = h
+        self.blocks: list[Block] = []
+
+    def add(self, b: "Block") -> None:
+        _check(_lib.cfdx_batch_add(self.h, b.h))
+        self.blocks.append(b)
+
+    def remove(self, b: "Block") -> None:
+        _check(_lib.cfdx_batch_remove(self.h, b.h))
+        self.blocks.remove(b)
+
+    def __len__(self) -> int:
+        return int(_lib.cfdx_batch_size(self.h))
+
+    def set_grid(self, cap: int) -> None:
+        _check(_lib.cfdx_batch_set_grid(self.h, cap))
+
+    def step(self, nsteps: int, graph: bool = True) -> None:
+        _check(_lib.cfdx_batch_step(self.h, nsteps, int(graph)))
+
+    def close(self) -> None:
+        if self.h:
+            _check(_lib.cfdx_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def exported_symbols() -> list[str]:

@@ -1,0 +1,57 @@
+// Structured-block data shared by the block kernels and the block / batch host objects.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ctx.hpp"
+#include "physics.hpp"
+
+namespace cfdx {
+namespace blk {
+
+constexpr int NODE_TX = 64, NODE_TY = 4;   // node kernels: 64 x 4 tile at one k
+
+// face tiles: TB faces along the sweep direction times TA lines across it
+template <int DIR> struct FaceTile { static constexpr int TA = 16, TB = 16; };   // across = x, along = y or z
+template <> struct FaceTile<0> { static constexpr int TA = 4, TB = 64; };      // along = x, across = y
+
+// Device-side view of one block: sizes, array pointers and per-block run parameters.
+// P, Fv and FH hold the flux type (double or float); everything else is double.
+struct BlockDev {
+    int ni, nj, nk;
+    uint32_t n;
+    uint32_t tile0[4];          // first global tile of this block: node tiles, face tiles x, y, z
+    const double *det;          // det at the node
+    const double *m[9];         // node metrics zeta_x .. xi_z
+    const double *kf[3][3];     // face metric row of direction d at the +1/2 face (zeta_?ip, eta_?jp, xi_?kp)
+    const double *df[3];        // half-point det at ip, jp, kp
+    void *P[5];                 // rho, u, v, w, p used by the flux kernels
+    double *U0[5];              // RK base det*Q
+    double *Us[5];              // stage vector U1 / U2
+    void *Fv[3][4];             // viscous fluxes, components 1..4 (the mass component is identically 0)
+    double *mu;
+    void *FH[3][5];             // face fluxes F-hat at the +1/2 faces
+    double *Lout;               // RHS output (mode 3)
+    double del_t, mach, eps;
+    int viscous;
+    cfdx::dev::ViscParams vp;
+};
+
+// One launch over a list of blocks.
+struct LaunchList {
+    const BlockDev *blks = nullptr;   // device array
+    const uint32_t *prefix[4] = {};   // device arrays of nblk+1 tile offsets (node, face x, y, z)
+    uint32_t nblk = 0;
+    uint32_t total[4] = {};
+    uint32_t grid_cap = 0;            // 0: one workgroup per tile; else persistent grid of this size
+};
+
+// kernels (block_kernels.hip)
+void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev);
+void base(const BlockDev &B, int precision, hipStream_t st);
+void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
+uint32_t tiles_of(const BlockDev &B, int kind);
+
+}  // namespace blk
+}  // namespace cfdx

@@ -1,0 +1,391 @@
+// Structured-block kernels. Each block is one periodic ni x nj x nk grid held in HBM as
+// structure-of-arrays; a launch covers a list of blocks (one for cfdx_block_step, many for a
+// batch), every workgroup taking one tile of one block through a prefix table of tile counts.
+//   k_visc        node kernel: Fv, Ev, Gv (+ mu) from the stage vector U (Viscous.cl)
+//   k_face<DIR>   one thread per face (i+1/2 along DIR): characteristic WCNS reconstruction,
+//                 LLF and the 6th-order midpoint correction (Weno.cl), LDS-staged tiles
+//   k_update      flux differences + viscous divergence -> L, SSP-RK3 stage j, primitives
+//                 (Weno.cl:3060-3135)
+// Each face flux is computed once and shared by its two nodes (the reference computes it
+// twice, once from each side, with identical operands).
+// S is the flux arithmetic / storage type: double (plain form) or float (mixed form). The RK
+// base U0, the stage vector Us, the metrics and the L accumulation are double in both.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "block.hpp"
+
+using namespace cfdx::dev;
+
+namespace cfdx {
+namespace blk {
+
+// Loads and stores at a 32-bit element index from a uniform base pointer, so the compiler
+// emits SGPR-base + VGPR-offset global accesses (a block holds < 2^29 nodes).
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, uint32_t i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, uint32_t i, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__device__ __forceinline__ uint32_t node_index(const BlockDev &B, int i, int j, int k)
+{
+    return (uint32_t)i + (uint32_t)B.ni * ((uint32_t)j + (uint32_t)B.nj * (uint32_t)k);
+}
+
+__device__ __forceinline__ uint32_t shifted(const BlockDev &B, int i, int j, int k, int d, int o)
+{
+    if (d == 0) i = wrapi(i + o, B.ni);
+    else if (d == 1) j = wrapi(j + o, B.nj);
+    else k = wrapi(k + o, B.nk);
+    return node_index(B, i, j, k);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each
+// with its own L2. Remap the linear workgroup id so XCD x takes one contiguous range of tiles
+// (a slab of k planes of one block). Bijective for any count.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
+    return x * q + (x < r ? x : r) + l;
+}
+
+// block of a global tile id: the largest b with prefix[b] <= t (prefix[0] = 0, prefix[nblk] = total)
+__device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ prefix, uint32_t nblk, uint32_t t)
+{
+    uint32_t lo = 0, hi = nblk;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// viscous node kernel (Viscous.cl:285-631 on a periodic block: central 4th order)
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(NODE_TX *NODE_TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+                                                           uint32_t nblk, uint32_t total, int mode)
+{
+    for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
+        const uint32_t t = xcd_tile(wg, total);
+        const BlockDev &B = blks[find_block(prefix, nblk, t)];
+        if (!B.viscous) continue;
+        const uint32_t lt = t - B.tile0[0];
+        const uint32_t ntx = (B.ni + NODE_TX - 1) / NODE_TX, nty = (B.nj + NODE_TY - 1) / NODE_TY;
+        const int i = (int)((lt % ntx) * NODE_TX + threadIdx.x % NODE_TX);
+        const int j = (int)(((lt / ntx) % nty) * NODE_TY + threadIdx.x / NODE_TX);
+        const int k = (int)(lt / (ntx * nty));
+        if (i >= B.ni || j >= B.nj) continue;
+        const uint32_t g = node_index(B, i, j, k);
+        // stage vector into the viscous kernel: U0 at j = 0, the raw U1 / U2 after (Solver.cpp:427-435)
+        const double *const *Uin = (mode == 0 || mode == 3) ? B.U0 : B.Us;
+        double U[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) U[c] = ld(Uin[c], g);
+        double dU[5][3], KS[3];
+        const double iU0 = 1.0 / U[0];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
+            const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+            double um1[5], up1[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double *u = Uin[c];
+                um1[c] = ld(u, m1);
+                up1[c] = ld(u, p1);
+                dU[c][d] = (1.0 / 12.0) * (8.0 * (up1[c] - um1[c]) - (ld(u, p2) - ld(u, m2)));
+            }
+            // MTS scale-similarity term of momentum component d+1 along d (Viscous.cl:493-495)
+            const double a = (U[1 + d] * iU0) - ((0.25 * um1[1 + d] + 0.5 * U[1 + d] + 0.25 * up1[1 + d]) / (0.25 * um1[0] + 0.5 * U[0] + 0.25 * up1[0]));
+            KS[d] = a * a;
+        }
+        // the differences above are taken in double; the flux arithmetic runs in S
+        S dUs[5][3], Uv[5], KSs[3], m[9];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            Uv[c] = S(U[c]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) dUs[c][d] = S(dU[c][d]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) KSs[d] = S(KS[d]);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) m[c] = S(ld(B.m[c], g));
+        S fv[3][5];
+        const S mu = viscous_node(dUs, Uv, ld((const S *)B.P[1], g), ld((const S *)B.P[2], g), ld((const S *)B.P[3], g), m,
+                                  S(ld(B.det, g)), KSs, B.vp, fv);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 1; c < 5; ++c) st((S *)B.Fv[d][c - 1], g, fv[d][c]);
+        if (mode >= 2) st(B.mu, g, (double)mu);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// face kernel: F-hat at the +1/2 face of every node along DIR. A tile is TB faces along DIR
+// times TA lines across it. Phase 1 stages the conserved state Q and the node flux F_DIR of the
+// TB+5 stencil points of every line in LDS (each point computed once per tile); phase 2 gives
+// each thread one face, reading its six stencil points from LDS.
+// ---------------------------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ int lds_at(int c, int s, int a)
+{
+    constexpr int TA = FaceTile<DIR>::TA, NP = FaceTile<DIR>::TB + 5;
+    return DIR == 0 ? (c * TA + a) * NP + s : (c * NP + s) * TA + a;
+}
+
+template <int DIR>
+__device__ __forceinline__ uint32_t tile_node(const BlockDev &B, int along, int across, int fixed)
+{
+    if (DIR == 0) return node_index(B, along, across, fixed);
+    if (DIR == 1) return node_index(B, across, along, fixed);
+    return node_index(B, across, fixed, along);
+}
+
+template <typename S, int DIR>
+__global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+                                              uint32_t nblk, uint32_t total)
+{
+    constexpr int TA = FaceTile<DIR>::TA, TB = FaceTile<DIR>::TB, NP = TB + 5;
+    __shared__ S sq[5 * NP * TA];
+    __shared__ S sf[5 * NP * TA];
+    const int tid = threadIdx.x;
+    for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
+        const uint32_t t = xcd_tile(wg, total);
+        const BlockDev &B = blks[find_block(prefix, nblk, t)];
+        const uint32_t lt = t - B.tile0[1 + DIR];
+        const int n_along = DIR == 0 ? B.ni : DIR == 1 ? B.nj : B.nk;
+        const int n_across = DIR == 0 ? B.nj : B.ni;
+        const uint32_t nta = (n_across + TA - 1) / TA, ntb = (n_along + TB - 1) / TB;
+        const uint32_t nfix = DIR == 2 ? B.nj : B.nk;
+        int b0, a0, fixed;
+        if (DIR == 0) { b0 = (lt % ntb) * TB; a0 = ((lt / ntb) % nta) * TA; fixed = lt / (ntb * nta); }
+        else if (DIR == 1) { a0 = (lt % nta) * TA; b0 = ((lt / nta) % ntb) * TB; fixed = lt / (nta * ntb); }
+        else { a0 = (lt % nta) * TA; fixed = (lt / nta) % nfix; b0 = (lt / (nta * nfix)) * TB; }
+        const S *P0 = (const S *)B.P[0], *P1 = (const S *)B.P[1], *P2 = (const S *)B.P[2], *P3 = (const S *)B.P[3], *P4 = (const S *)B.P[4];
+
+        __syncthreads();   // the previous tile's LDS fully consumed
+        for (int p = tid; p < NP * TA; p += 256) {
+            const int s = DIR == 0 ? p % NP : p / TA;
+            const int a = DIR == 0 ? p / NP : p % TA;
+            if (a0 + a >= n_across) continue;
+            const int along = wrapi(b0 + s - 2, n_along);
+            const uint32_t x = tile_node<DIR>(B, along, a0 + a, fixed);
+            const S rho = ld(P0, x), u = ld(P1, x), v = ld(P2, x), w = ld(P3, x), pr = ld(P4, x);
+            S q[5], f[5];
+            prim_to_q(rho, u, v, w, pr, q);
+            node_flux(rho, u, v, w, pr, S(ld(B.m[3 * DIR], x)), S(ld(B.m[3 * DIR + 1], x)), S(ld(B.m[3 * DIR + 2], x)),
+                      S(ld(B.det, x)), f);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                sq[lds_at<DIR>(c, s, a)] = q[c];
+                sf[lds_at<DIR>(c, s, a)] = f[c];
+            }
+        }
+        __syncthreads();
+
+        const int b = DIR == 0 ? tid % TB : tid / TA;
+        const int a = DIR == 0 ? tid / TB : tid % TA;
+        if (b0 + b < n_along && a0 + a < n_across) {
+            S q[6][5];
+#pragma unroll
+            for (int m = 0; m < 6; ++m)
+#pragma unroll
+                for (int c = 0; c < 5; ++c) q[m][c] = sq[lds_at<DIR>(c, b + m, a)];
+            const uint32_t gl = tile_node<DIR>(B, b0 + b, a0 + a, fixed);
+            const uint32_t gr = tile_node<DIR>(B, wrapi(b0 + b + 1, n_along), a0 + a, fixed);
+            S roe[5];
+            {
+                const S ra = ld(P0, gl), ua = ld(P1, gl), va = ld(P2, gl), wa = ld(P3, gl), pa = ld(P4, gl);
+                const S rb = ld(P0, gr), ub = ld(P1, gr), vb = ld(P2, gr), wb = ld(P3, gr), pb = ld(P4, gr);
+                roe_average(ra, ua, va, wa, enthalpy(ra, ua, va, wa, pa), rb, ub, vb, wb, enthalpy(rb, ub, vb, wb, pb), roe);
+            }
+            S llf[5];
+            face_llf(q, roe, S(ld(B.kf[DIR][0], gl)), S(ld(B.kf[DIR][1], gl)), S(ld(B.kf[DIR][2], gl)), S(ld(B.df[DIR], gl)),
+                     S(B.eps), llf);
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                st((S *)B.FH[DIR][c], gl,
+                   S(llf[c] + midpoint_correction(sf[lds_at<DIR>(c, b, a)], sf[lds_at<DIR>(c, b + 1, a)],
+                                                  sf[lds_at<DIR>(c, b + 2, a)], sf[lds_at<DIR>(c, b + 3, a)],
+                                                  sf[lds_at<DIR>(c, b + 4, a)], sf[lds_at<DIR>(c, b + 5, a)])));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update kernel: L and the SSP-RK3 stage (Weno.cl:3060-3135), in double.
+//   mode 0..2: RK stage j; mode 3: write L to the block's Lout only
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(NODE_TX *NODE_TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+                                                             uint32_t nblk, uint32_t total, int mode)
+{
+    for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
+        const uint32_t t = xcd_tile(wg, total);
+        const BlockDev &B = blks[find_block(prefix, nblk, t)];
+        const uint32_t lt = t - B.tile0[0];
+        const uint32_t ntx = (B.ni + NODE_TX - 1) / NODE_TX, nty = (B.nj + NODE_TY - 1) / NODE_TY;
+        const int i = (int)((lt % ntx) * NODE_TX + threadIdx.x % NODE_TX);
+        const int j = (int)(((lt / ntx) % nty) * NODE_TY + threadIdx.x / NODE_TX);
+        const int k = (int)(lt / (ntx * nty));
+        if (i >= B.ni || j >= B.nj) continue;
+        const uint32_t g = node_index(B, i, j, k);
+        double L[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t m1 = shifted(B, i, j, k, d, -1);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const S *fh = (const S *)B.FH[d][c];
+                L[c] += (double)ld(fh, g) - (double)ld(fh, m1);
+            }
+        }
+        if (B.viscous) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
+                const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+#pragma unroll
+                for (int c = 1; c < 5; ++c) {
+                    const S *f = (const S *)B.Fv[d][c - 1];
+                    L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) L[c] = -L[c];
+        if (mode == 3) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) B.Lout[(size_t)c * B.n + g] = L[c];
+            continue;
+        }
+        const double del_t = B.del_t;
+        double un[5];
+        if (mode == 0) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) { un[c] = ld(B.U0[c], g) + del_t * L[c]; st(B.Us[c], g, un[c]); }
+        } else if (mode == 1) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) { un[c] = 0.75 * ld(B.U0[c], g) + 0.25 * ld(B.Us[c], g) + 0.25 * del_t * L[c]; st(B.Us[c], g, un[c]); }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) un[c] = (ld(B.U0[c], g) * (1.0 / 3.0)) + ((2.0 / 3.0) * ld(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
+        }
+        const double det = ld(B.det, g);
+        const double idet = 1.0 / det;
+        const double rho = idet * un[0], ir = 1.0 / rho;
+        const double u = (idet * un[1]) * ir, v = (idet * un[2]) * ir, w = (idet * un[3]) * ir;
+        const double e = (idet * un[4]) * ir - 0.5 * (u * u + v * v + w * w);
+        const double p = e * rho * 0.4;
+        st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
+        if (mode == 2) {
+            // next step's RK base: U_Q recomputes det*Q from the new primitives (Solver.cpp:377-388)
+            double q[5];
+            prim_to_q(rho, u, v, w, p, q);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
+        }
+    }
+}
+
+// set_flow: double primitives staged in Us -> P (type S) and U0 = det * Q(P) in double
+template <typename S>
+__global__ void k_base(BlockDev B)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= B.n) return;
+    const double rho = ld(B.Us[0], g), u = ld(B.Us[1], g), v = ld(B.Us[2], g), w = ld(B.Us[3], g), p = ld(B.Us[4], g);
+    double q[5];
+    prim_to_q(rho, u, v, w, p, q);
+    const double det = ld(B.det, g);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
+    st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
+}
+
+// get_flow of the mixed form: primitives from the double RK base U0 / det
+__global__ void k_prims(BlockDev B, double *__restrict__ out)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= B.n) return;
+    const double idet = 1.0 / ld(B.det, g);
+    const double rho = idet * ld(B.U0[0], g), ir = 1.0 / rho;
+    const double u = (idet * ld(B.U0[1], g)) * ir, v = (idet * ld(B.U0[2], g)) * ir, w = (idet * ld(B.U0[3], g)) * ir;
+    const double e = (idet * ld(B.U0[4], g)) * ir - 0.5 * (u * u + v * v + w * w);
+    out[g] = rho; out[B.n + g] = u; out[2 * (size_t)B.n + g] = v; out[3 * (size_t)B.n + g] = w; out[4 * (size_t)B.n + g] = e * rho * 0.4;
+}
+
+// get_flow of the plain form: the primitives of the last stage
+__global__ void k_copy_prims(BlockDev B, double *__restrict__ out)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= B.n) return;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) out[(size_t)c * B.n + g] = ld((const double *)B.P[c], g);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <typename S>
+void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev)
+{
+    auto rec = [&](int s) { if (ev) (void)hipEventRecord(ev[s], st); };
+    auto grid = [&](uint32_t total) { return dim3(L.grid_cap ? std::min(total, L.grid_cap) : total); };
+    rec(0);
+    if (any_visc) hipLaunchKernelGGL((k_visc<S>), grid(L.total[0]), dim3(NODE_TX * NODE_TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    rec(1);
+    hipLaunchKernelGGL((k_face<S, 0>), grid(L.total[1]), dim3(256), 0, st, L.blks, L.prefix[1], L.nblk, L.total[1]);
+    hipLaunchKernelGGL((k_face<S, 1>), grid(L.total[2]), dim3(256), 0, st, L.blks, L.prefix[2], L.nblk, L.total[2]);
+    hipLaunchKernelGGL((k_face<S, 2>), grid(L.total[3]), dim3(256), 0, st, L.blks, L.prefix[3], L.nblk, L.total[3]);
+    rec(2);
+    hipLaunchKernelGGL((k_update<S>), grid(L.total[0]), dim3(NODE_TX * NODE_TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    rec(3);
+}
+
+void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev)
+{
+    if (precision == CFDX_MIXED) launch_stage<float>(L, mode, st, any_visc, ev);
+    else launch_stage<double>(L, mode, st, any_visc, ev);
+}
+
+void base(const BlockDev &B, int precision, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((B.n + 255) / 256);
+    if (precision == CFDX_MIXED) hipLaunchKernelGGL(k_base<float>, dim3(nb), dim3(256), 0, st, B);
+    else hipLaunchKernelGGL(k_base<double>, dim3(nb), dim3(256), 0, st, B);
+}
+
+void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
+{
+    const unsigned nb = (unsigned)((B.n + 255) / 256);
+    if (precision == CFDX_MIXED) hipLaunchKernelGGL(k_prims, dim3(nb), dim3(256), 0, st, B, out);
+    else hipLaunchKernelGGL(k_copy_prims, dim3(nb), dim3(256), 0, st, B, out);
+}
+
+uint32_t tiles_of(const BlockDev &B, int kind)
+{
+    if (kind == 0) return (uint32_t)(((B.ni + NODE_TX - 1) / NODE_TX) * ((B.nj + NODE_TY - 1) / NODE_TY) * B.nk);
+    auto f = [&](int TA, int TB, int along, int across, int fix) {
+        return (uint32_t)(((across + TA - 1) / TA) * ((along + TB - 1) / TB) * fix);
+    };
+    if (kind == 1) return f(FaceTile<0>::TA, FaceTile<0>::TB, B.ni, B.nj, B.nk);
+    if (kind == 2) return f(FaceTile<1>::TA, FaceTile<1>::TB, B.nj, B.ni, B.nk);
+    return f(FaceTile<2>::TA, FaceTile<2>::TB, B.nk, B.ni, B.nj);
+}
+
+}  // namespace blk
+}  // namespace cfdx

@@ -24,7 +24,7 @@ extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
     for (auto &ev : c->ev) {
         hipError_t e = hipEventCreate(&ev);
         if (e != hipSuccess) {
-            cfdx_ctx_destroy(c);
+            cfdx::ctx_release(c);
             return cfdx::hip_fail(e, "hipEventCreate");
         }
     }
@@ -32,9 +32,15 @@ extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
     return CFDX_OK;
 }
 
-extern "C" int cfdx_ctx_destroy(cfdx_ctx *ctx)
+namespace cfdx {
+void ctx_retain(cfdx_ctx *ctx)
 {
-    if (!ctx) return CFDX_OK;
+    if (ctx) ctx->refs.fetch_add(1);
+}
+
+int ctx_release(cfdx_ctx *ctx)
+{
+    if (!ctx || ctx->refs.fetch_sub(1) > 1) return CFDX_OK;
     int rc = CFDX_OK;
     if (hipSetDevice(ctx->device) == hipSuccess && ctx->owns_stream) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -46,6 +52,9 @@ extern "C" int cfdx_ctx_destroy(cfdx_ctx *ctx)
     delete ctx;
     return rc;
 }
+}  // namespace cfdx
+
+extern "C" int cfdx_ctx_destroy(cfdx_ctx *ctx) { return cfdx::ctx_release(ctx); }
 
 extern "C" int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream)
 {

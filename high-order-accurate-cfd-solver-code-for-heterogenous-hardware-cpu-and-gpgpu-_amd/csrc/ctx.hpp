@@ -2,14 +2,24 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "common.hpp"
 
+// A context is shared by the blocks and batches made on it: each holds a reference, and the
+// context is freed when cfdx_ctx_destroy and the last holder have released it.
 struct cfdx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = false;
     hipEvent_t ev[8] = {};
+    std::atomic<int> refs{1};
 };
+
+namespace cfdx {
+void ctx_retain(cfdx_ctx *ctx);
+int ctx_release(cfdx_ctx *ctx);
+}  // namespace cfdx
 
 namespace cfdx {
 

@@ -10,9 +10,18 @@ namespace dev {
 
 template <typename T> __device__ __forceinline__ T sqrt_(T x);
 template <> __device__ __forceinline__ double sqrt_<double>(double x) { return __builtin_sqrt(x); }
-template <> __device__ __forceinline__ float sqrt_<float>(float x) { return __builtin_sqrtf(x); }
+template <> __device__ __forceinline__ float sqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename T> __device__ __forceinline__ T fabs_(T x) { return x < T(0) ? -x : x; }
+
+// Reciprocal and quotient: exact IEEE division in double (the plain form's parity bar), the
+// hardware reciprocal / square root (about 1 ulp) in float for the mixed form.
+template <typename T> __device__ __forceinline__ T inv_(T x);
+template <> __device__ __forceinline__ double inv_<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float inv_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename T> __device__ __forceinline__ T div_(T a, T b);
+template <> __device__ __forceinline__ double div_<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float div_<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Conserved state Q = (rho, rho u, rho v, rho w, p/0.4 + rho |u|^2 / 2) of primitives (U_Q.cl:118-122).
 template <typename T>
@@ -30,7 +39,7 @@ __device__ __forceinline__ void prim_to_q(T rho, T u, T v, T w, T p, T (&q)[5])
 template <typename T>
 __device__ __forceinline__ T enthalpy(T rho, T u, T v, T w, T p)
 {
-    return (p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w)) / rho;   // (p/0.4 + ke + p) / rho
+    return div_(p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w), rho);   // (p/0.4 + ke + p) / rho
 }
 
 // Roe average of the face between states a and b (U_Q.cl:60-109): (u, v, w, h, rho).
@@ -38,7 +47,7 @@ template <typename T>
 __device__ __forceinline__ void roe_average(T ra, T ua, T va, T wa, T ha, T rb, T ub, T vb, T wb, T hb, T (&r)[5])
 {
     const T sa = sqrt_(ra), sb = sqrt_(rb);
-    const T is = T(1) / (sa + sb);
+    const T is = inv_(sa + sb);
     r[0] = (sb * ub + sa * ua) * is;
     r[1] = (sb * vb + sa * va) * is;
     r[2] = (sb * wb + sa * wa) * is;
@@ -93,25 +102,37 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     const T ISm0 = c13 * s2 * s2 + c14 * u0 * u0;
     const T ISm1 = c13 * s1 * s1 + c14 * u1 * u1;
     const T ISm2 = c13 * s0 * s0 + c14 * u2 * u2;
-    // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2; one reciprocal serves both sides
-    T np_, dp_, nn_, dn_;
-    {
-        T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
-        a0 *= a0; a1 *= a1; a2 *= a2;
-        const T w0 = T(1.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(5.0 / 16.0) * (a0 * a1);
-        np_ = w0 * h_r + w1 * h_c + w2 * h_l;
-        dp_ = w0 + w1 + w2;
+    if constexpr (sizeof(T) == sizeof(double)) {
+        // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2: one division serves both sides
+        T np_, dp_, nn_, dn_;
+        {
+            T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
+            a0 *= a0; a1 *= a1; a2 *= a2;
+            const T w0 = T(1.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(5.0 / 16.0) * (a0 * a1);
+            np_ = w0 * h_r + w1 * h_c + w2 * h_l;
+            dp_ = w0 + w1 + w2;
+        }
+        {
+            T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
+            a0 *= a0; a1 *= a1; a2 *= a2;
+            const T w0 = T(5.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(1.0 / 16.0) * (a0 * a1);
+            nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
+            dn_ = w0 + w1 + w2;
+        }
+        const T r = inv_(dp_ * dn_);
+        pos = np_ * (dn_ * r);
+        neg = nn_ * (dp_ * r);
+    } else {
+        // float: the products above leave the float range on smooth data (eps^8 ~ 1e-56);
+        // w_k = d_k / (eps + IS_k)^2 with hardware reciprocals stays within it
+        auto side = [&](T i0, T i1, T i2, T d0, T d1, T d2, T q0, T q1, T q2) {
+            T a0 = eps + i0, a1 = eps + i1, a2 = eps + i2;
+            const T w0 = d0 * inv_(a0 * a0), w1 = d1 * inv_(a1 * a1), w2 = d2 * inv_(a2 * a2);
+            return (w0 * q0 + w1 * q1 + w2 * q2) * inv_(w0 + w1 + w2);
+        };
+        pos = side(ISp0, ISp1, ISp2, T(1.0 / 16.0), T(5.0 / 8.0), T(5.0 / 16.0), h_r, h_c, h_l);
+        neg = side(ISm0, ISm1, ISm2, T(5.0 / 16.0), T(5.0 / 8.0), T(1.0 / 16.0), h_c, h_l, h_ll);
     }
-    {
-        T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
-        a0 *= a0; a1 *= a1; a2 *= a2;
-        const T w0 = T(5.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(1.0 / 16.0) * (a0 * a1);
-        nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
-        dn_ = w0 + w1 + w2;
-    }
-    const T r = T(1) / (dp_ * dn_);
-    pos = np_ * (dn_ * r);
-    neg = nn_ * (dp_ * r);
 }
 
 // Characteristic reconstruction of the two interface states of one face
@@ -124,14 +145,14 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
                                                       T (&qp)[5], T (&qn)[5])
 {
     const T u = roe[0], v = roe[1], w = roe[2], h = roe[3], rho = roe[4];
-    const T ikk = T(1) / sqrt_(k0 * k0 + k1 * k1 + k2 * k2);
+    const T ikk = inv_(sqrt_(k0 * k0 + k1 * k1 + k2 * k2));
     const T kx = k0 * ikk, ky = k1 * ikk, kz = k2 * ikk;
     const T q2 = u * u + v * v + w * w;
     const T a = sqrt_(T(0.4) * (h - T(0.5) * q2));
     const T theta = kx * u + ky * v + kz * w;
     const T phi = T(0.2) * q2;
-    const T irho = T(1) / rho;
-    const T ia = T(1) / a;
+    const T irho = inv_(rho);
+    const T ia = inv_(a);
     const T iaa = ia * ia;
     const T s2 = T(0.70710678118654752440);           // 1/sqrt(2)
     const T alpha = rho * ia * s2;                      // rho / (sqrt2 a)
@@ -182,7 +203,7 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
 template <typename T>
 __device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k2, T kk2, T detf, T (&c)[5], T &lam)
 {
-    const T r = T(1) / Q[0];
+    const T r = inv_(Q[0]);
     const T ke = (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * r;
     const T P = T(0.4) * (Q[4] - T(0.5) * ke);
     const T Un = (k0 * Q[1] + k1 * Q[2] + k2 * Q[3]) * r;
@@ -273,7 +294,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
                                           const T (&KS)[3], const ViscParams &vp, T (&fvis)[3][5])
 {
     const T C_MTS = T(vp.c_mts);
-    const T iU0 = T(1) / U[0];
+    const T iU0 = inv_(U[0]);
     const T iU02 = iU0 * iU0;
     T ug[3], vg[3], wg[3], tg[3];
 #pragma unroll
@@ -302,7 +323,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     // Viscous.cl:455-458: T = gamma M^2 (gamma-1) e, Sutherland mu with S = 110.4
     const T d_temp = T(vp.tfac) * (U[4] * iU0 - T(0.5) * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * iU02));
     const T st = sqrt_(d_temp);
-    const T mu = (st * st * st) * (T(vp.suth1) / (d_temp + T(vp.suth)));
+    const T mu = (st * st * st) * div_(T(vp.suth1), d_temp + T(vp.suth));
     // Viscous.cl:461-476 heat flux and Stokes stresses
     const T kq = -(mu * T(vp.kheat));
     const T qx = kq * ex, qy = kq * ey, qz = kq * ez;
@@ -317,7 +338,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     const T sxy = T(0.5) * (uy + vx), syz = T(0.5) * (vz + wy), sxz = T(0.5) * (wx + uz);
     const T sxx = ux, syy = vy, szz = wz;
     const T mod = sqrt_(T(2) * ((sxy * sxy + syz * syz + sxz * sxz) + T(2.0 / 3.0) * (sxx * sxx + syy * syy + szz * szz)));
-    const T DEL = T(1) / cbrt_(D);
+    const T DEL = inv_(cbrt_(D));
     T eddy[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -325,7 +346,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
         if (mod != T(0)) {
             // C_MTS / (1 + R / C_T) * DEL * sk with R = DEL |S| / sk, written with one division
             // (sk = 0 gives R = inf and a zero eddy viscosity either way)
-            eddy[d] = (C_MTS * DEL * sk * sk) / (sk + DEL * mod * T(vp.inv_ct));
+            eddy[d] = div_(C_MTS * DEL * sk * sk, sk + DEL * mod * T(vp.inv_ct));
         } else {
             eddy[d] = C_MTS * DEL * sk;
         }

@@ -145,7 +145,7 @@ typedef struct cfdx_block cfdx_block;
 typedef struct {
     int32_t ni, nj, nk;   /* nodes along i (zeta), j (eta), k (xi); node (i,j,k) is i + ni*(j + nj*k) */
     int32_t periodic;     /* must be 7 (periodic in i, j and k) */
-    int32_t precision;    /* CFDX_FP64 or CFDX_MIXED */
+    int32_t precision;    /* CFDX_FP64, or CFDX_MIXED (float fluxes, double RK state and metrics) */
 } cfdx_block_desc;
 
 /* Bytes of device memory a block of this shape holds. */
@@ -158,7 +158,11 @@ int cfdx_block_set_geometry(cfdx_block *b, const double *det, const cfdx_metric 
 int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow);
 int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow);
 
-/* nsteps SSP-RK3 steps with fixed del_t; Mach is the Weno-argument Mach (t = gamma M^2 p/rho).
+/* Run parameters of the block (Weno-argument Mach for t = gamma M^2 p/rho, fixed del_t, physical
+ * constants, viscous switch), used by cfdx_block_step and by batched steps. */
+int cfdx_block_configure(cfdx_block *b, double Mach, double del_t, const cfdx_params *params);
+
+/* nsteps SSP-RK3 steps with fixed del_t (configures the block with these arguments first).
  * Asynchronous on the context stream. */
 int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nsteps, const cfdx_params *params);
 
@@ -170,6 +174,22 @@ int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *L_host);
  * measured with HIP events on the context stream when timing is on. */
 int cfdx_block_set_timing(cfdx_block *b, int on);
 int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face, double *ms_update, int *launches);
+
+/* Batches: blocks of different sizes and run parameters stepped together. Each kernel class is
+ * one launch over the tiles of every block; a step can be replayed from a captured hipGraph.
+ * A batch references its blocks (it does not own them); all share its precision and device. */
+typedef struct cfdx_batch cfdx_batch;
+int cfdx_batch_create(cfdx_ctx *ctx, int precision, cfdx_batch **out);
+int cfdx_batch_destroy(cfdx_batch *q);
+int cfdx_batch_add(cfdx_batch *q, cfdx_block *b);      /* the block must be configured */
+int cfdx_batch_remove(cfdx_batch *q, cfdx_block *b);
+int cfdx_batch_size(cfdx_batch *q);
+/* grid_cap = 0: one workgroup per tile; > 0: persistent launches of at most grid_cap workgroups
+ * that loop over the tiles. */
+int cfdx_batch_set_grid(cfdx_batch *q, uint32_t grid_cap);
+/* nsteps RK3 steps of every block; use_graph = 1 captures one step in a hipGraph on first use
+ * (re-captured after the batch changes) and replays it. Asynchronous on the context stream. */
+int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph);
 
 /* ------------------------------------------------------------------------ */
 /* 3. Host geometry (CPU)                                                    */

@@ -50,6 +50,8 @@ def _compare(a, b, rows, keys=None, tol=TOL):
         if getattr(x, "dtype", None) is not None and x.dtype.names:
             x, y = cases.flow_matrix(x), cases.flow_matrix(y)
         x, y = x[rows], y[rows]
+        if len(x) == 0:
+            continue
         if k == "ROE":
             keep = [i for i in range(36) if i % 6 != 3]  # roe_p is never written
             x, y = x[:, keep], y[:, keep]
@@ -83,6 +85,7 @@ def test_dropin_stage_walls(ctx, orc, walls):
     vmask = cases.defined_mask(case, weno_form=False) & core
     _compare(a, b, vmask, keys=["F", "E", "G", "Fv", "Ev", "Gv", "mu"])
     wmask = cases.defined_mask(case, weno_form=True) & vmask
+    assert vmask.sum() > 0 and (wmask.sum() > 300 or walls == (True, True, True))
     _compare(a, b, wmask, keys=["U1", "FLOW"])
 
 
