@@ -23,8 +23,13 @@ PKG_DIR = os.path.join(ROOT, "high-order-accurate-cfd-solver-code-for-heterogeno
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD
+VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 ALG_BYTES_PER_CELL_UPDATE = 848  # minimal FP64 SoA traffic of one RK3 step (3 stages)
-FACE_BYTES_PER_NODE = 144        # k_face: reads P(5) + metric row(3) + det + face k(3) + face det, writes F-hat(5)
+# k_face per face: reads the 5 primitives P and writes the 5 face fluxes F-hat in the flux type
+# (8 B FP64, 4 B mixed) and reads 8 FP64 geometry values (metric row 3, det, face normal 3, face det)
+def face_bytes_per_node(precision: str) -> int:
+    return 64 + 10 * (8 if precision == "fp64" else 4)
 
 
 def _import_pkg():
@@ -53,6 +58,32 @@ def workload(name: str):
         case = dict(ni=64, nj=64, nk=64, kw=dict(grid="box", length=(10.0,) * 3, init="vortex", mach=0.5))
         return spec, case, dict(), False
     raise SystemExit(f"unknown workload {name}")
+
+
+def profile_counts(workload_name: str, precision: str):
+    """Per-launch HBM bytes and VALU counts of the face kernels, from the committed rocprofv3
+    summary of this workload (profiles/block_kernels_<workload>_<precision>.md). The summary
+    holds one machine-readable row per kernel:
+    `| kernel | launches | avg_us | hbm_bytes | valu_instr | fp64_flop | fp32_flop |`."""
+    path = os.path.join(ROOT, "profiles", f"block_kernels_{workload_name}_{precision}.md")
+    if not os.path.exists(path):
+        return None
+    rows = {}
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0].startswith("k_"):
+            try:
+                rows[cells[0]] = dict(launches=int(cells[1]), avg_us=float(cells[2]),
+                                      hbm_bytes=float(cells[3]), valu_instr=float(cells[4]),
+                                      fp64_flop=float(cells[5]), fp32_flop=float(cells[6]))
+            except ValueError:
+                continue
+    face = [v for k, v in rows.items() if k.startswith("k_face")]
+    if len(face) != 3:
+        return None
+    mean = {k: sum(f[k] for f in face) / 3 for k in ("avg_us", "hbm_bytes", "valu_instr", "fp64_flop", "fp32_flop")}
+    mean["source"] = os.path.relpath(path, ROOT)
+    return mean
 
 
 def cpu_baseline(cfdx, max_seconds: float = 15.0):
@@ -141,18 +172,34 @@ def main() -> int:
     block.set_timing(False)
     face_launches = 9 * args.profile_steps
     face_ms = tk["face_ms"] / face_launches
-    achieved = FACE_BYTES_PER_NODE * n / (face_ms / 1e3) / 1e9
+    face_bytes = face_bytes_per_node(args.precision)
+    achieved = face_bytes * n / (face_ms / 1e3) / 1e9
     flow = block.get_flow()
     finite = bool(np.isfinite(cfdx_matrix(flow)).all())
 
-    # reference copy kernel of known size (scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)
+    # reference copy kernel of known size (scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run);
+    # the first launch loads the code object, so time the second
     nb = args.copy_mib << 20
     a, b = ctx.zeros(nb // 8), ctx.zeros(nb // 8)
+    ctx.copy(b, a)
     ctx.record(2)
     ctx.copy(b, a)
     ctx.record(3)
     copy_ms = ctx.elapsed_ms(2, 3)
     a.free(); b.free()
+
+    prof = profile_counts(args.workload, args.precision)
+    traffic = prof["hbm_bytes"] if prof else None
+    valu = None
+    if prof:
+        sec = face_ms / 1e3
+        valu = {"wave_instr_per_launch": prof["valu_instr"],
+                "achieved_wave_instr_s": prof["valu_instr"] / sec,
+                "peak_wave_instr_s": VALU_PEAK_WAVE_INSTR_S,
+                "issue_frac": prof["valu_instr"] / sec / VALU_PEAK_WAVE_INSTR_S,
+                "fp64_tflops": prof["fp64_flop"] / sec / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                "fp32_tflops": prof["fp32_flop"] / sec / 1e12,
+                "counts_from": prof["source"], "profiled_kernel_us": prof["avg_us"]}
 
     line = {
         "metric": "cell-updates/s",
@@ -169,9 +216,10 @@ def main() -> int:
         "data": "synthetic",
         "config": dict(spec, parallelism="single GPU, one periodic block", precision=args.precision),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_face (one of 3 per stage)", "kernel_ms": face_ms,
-                     "bytes_per_launch": FACE_BYTES_PER_NODE * n},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_face (i, j, k sweeps; mean per launch)", "kernel_ms": face_ms,
+                     "bytes_per_launch": face_bytes * n},
+        "valu": valu,
         "hbm_alg_frac_whole_step": ALG_BYTES_PER_CELL_UPDATE * value / (HBM_PEAK_GBS * 1e9),
         "kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in tk.items() if k.endswith("_ms")},
         "state_finite": finite,
