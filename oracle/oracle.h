@@ -9,8 +9,9 @@
  * product (libcfdx) never links or calls it.
  *
  * The same entry points are exported by oracle/_ref/libref.so, which is the
- * reference's own kernel source (/root/reference/{U_Q,Viscous,Weno}.cl)
- * compiled as C by oracle/Makefile; tests run both on identical inputs.
+ * reference's own kernel source (U_Q.cl, Viscous.cl, Weno.cl) compiled as C
+ * by oracle/Makefile where the reference tree is present; tests run both on
+ * identical inputs.
  *
  * Struct layouts follow functions.h:49-112 byte for byte (packed).
  */
