@@ -6,7 +6,13 @@ from the initial state), full RHS (characteristic WCNS + LLF + 6th-order correct
 viscous/SGS fluxes) and RK3. A step = one full SSP-RK3 step over the whole grid with
 the state resident in HBM. Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1] [--no-cpu]
+Beside the headline the same JSON line carries (unless --legs drops them): the other precision
+on the same case with per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the 64-cube batched
+mixed-precision graph step ("batched_c4"), and a bounded sample of a Poisson request stream served
+by continuous batching vs plain FP64 one at a time ("serving": p50 / p99 latency, throughput).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1] [--precision fp64|mixed]
+                    [--legs compare,c4,serve] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -119,6 +125,127 @@ def cpu_baseline(cfdx, max_seconds: float = 15.0):
             "seconds": round(t3 - t2, 2)}
 
 
+def time_block(block, ctx, mach, dt, prm, steps, warmup, profile_steps):
+    """Warm-up, K timed steps bracketed by HIP events on the block's stream, then a separate
+    per-kernel-class timing pass."""
+    block.step(warmup, mach, dt, prm)
+    ctx.sync()
+    ctx.record(0)
+    block.step(steps, mach, dt, prm)
+    ctx.record(1)
+    ctx.sync()
+    dev_ms = ctx.elapsed_ms(0, 1)
+    block.set_timing(True)
+    block.step(profile_steps, mach, dt, prm)
+    tk = block.timing()
+    block.set_timing(False)
+    return dev_ms, tk
+
+
+def field_errors(a, b):
+    """Per-field relative L2 / Linf of a against b (rho, u, v, w, p); velocity components are
+    scaled by the norm of the whole velocity field."""
+    import numpy as np
+    vel = np.sqrt(sum(np.asarray(b[c]) ** 2 for c in "uvw"))
+    out = {}
+    for f in ("rho", "u", "v", "w", "p"):
+        d = a[f] - b[f]
+        ref = vel if f in "uvw" else np.abs(b[f])
+        out[f] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(ref)), "rel_linf": float(np.abs(d).max() / ref.max())}
+    return out
+
+
+C4_EDGES = (32, 48, 64, 96, 128)
+
+
+def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int = 64, grid_cap: int = 0):
+    """64 periodic cubes, edge U{32,48,64,96,128}, isentropic vortex / TGV ~ Bernoulli(1/2),
+    Mach U(0.1,0.6), Re U(500,5000), set up on the device; mixed precision, one batched launch per
+    kernel class replayed from a hipGraph of one RK3 step. Also times the same solves in plain
+    FP64 one block after another."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ctx = cfdx.Ctx(0)
+    specs = []
+    for _ in range(nblocks):
+        e = int(rng.choice(C4_EDGES))
+        tgv = bool(rng.integers(2))
+        specs.append(dict(e=e, tgv=tgv, mach=float(rng.uniform(0.1, 0.6)), reyl=float(rng.uniform(500, 5000)),
+                          phases=tuple(float(x) for x in rng.uniform(0, 2 * math.pi, 3))))
+
+    def build(precision):
+        blocks = []
+        for sp in specs:
+            e = sp["e"]
+            L = 2 * math.pi if sp["tgv"] else 10.0
+            b = cfdx.Block(e, e, e, precision=precision, ctx=ctx)
+            b.init_case(grid="sinusoid", length=(L,) * 3, amp=0.05 * L / e, phases=sp["phases"],
+                        init="tgv" if sp["tgv"] else "vortex", mach=sp["mach"], center=(5.0, 5.0))
+            dt = b.cfl_dt(0.5)
+            prm = cfdx.default_params(mach=sp["mach"], reyl=sp["reyl"])
+            b.configure(sp["mach"], dt, prm)
+            blocks.append((b, dt, prm, sp["mach"]))
+        return blocks
+
+    nodes = sum(sp["e"] ** 3 for sp in specs)
+    blocks = build(cfdx.MIXED)
+    q = cfdx.Batch(cfdx.MIXED, ctx)
+    for b, *_ in blocks:
+        q.add(b)
+    if grid_cap:
+        q.set_grid(grid_cap)
+    q.step(warmup, True)
+    ctx.sync()
+    ctx.record(0)
+    q.step(steps, True)
+    ctx.record(1)
+    ctx.sync()
+    ms = ctx.elapsed_ms(0, 1)
+    finite = all(bool(np.isfinite(cfdx_matrix(b.get_flow())).all()) for b, *_ in blocks[:4])
+    q.close()
+    for b, *_ in blocks:
+        b.close()
+    plain = build(cfdx.FP64)
+    for b, dt, prm, mach in plain:
+        b.step(warmup, mach, dt, prm)
+    ctx.sync()
+    ctx.record(2)
+    for b, dt, prm, mach in plain:
+        b.step(steps, mach, dt, prm)
+    ctx.record(3)
+    ctx.sync()
+    ms_plain = ctx.elapsed_ms(2, 3)
+    for b, *_ in plain:
+        b.close()
+    hist = {e: sum(1 for sp in specs if sp["e"] == e) for e in C4_EDGES}
+    return {"workload": "64 periodic cubes, edge U{32,48,64,96,128}, vortex/TGV, Mach U(0.1,0.6), Re U(500,5000)",
+            "edges": hist, "nodes": nodes, "steps": steps, "precision": "mixed", "graph": True,
+            "value": nodes * steps / (ms / 1e3), "unit": "cell-updates/s", "ms_per_step": ms / steps,
+            "plain_fp64_sequential": nodes * steps / (ms_plain / 1e3),
+            "speedup_vs_plain_fp64": ms_plain / ms, "state_finite": finite}
+
+
+def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202):
+    """A bounded sample of a Poisson request stream served by continuous batching (mixed) and by
+    plain FP64 one request at a time, on the same stream."""
+    from cfdx import serve
+    reqs = serve.periodic_stream(nreq, rate, seed=seed, nsteps=nsteps)
+    out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, grids 128x64x16..384x128x32, "
+                     "periodic TGV / vortex (curvilinear map), Mach U(0.1,0.6), Re U(500,5000)"}
+    for name, prec, batched in (("batched_mixed", cfdx.MIXED, True), ("plain_fp64", cfdx.FP64, False)):
+        srv = serve.Server(precision=prec, batched=batched, chunk=10, graph=True, keep_flow=True)
+        res = srv.run(reqs, clock="wall")
+        rep = serve.report(reqs, res)
+        rep["peak_active"] = srv.peak_active
+        rep["peak_bytes"] = srv.peak_bytes
+        srv.close()
+        out[name] = rep
+        print(f"bench: serving {name}: p50 {rep['p50_s']:.3f}s p99 {rep['p99_s']:.3f}s "
+              f"{rep['cell_updates_per_s']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    out["throughput_ratio"] = out["batched_mixed"]["cell_updates_per_s"] / out["plain_fp64"]["cell_updates_per_s"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +254,12 @@ def main() -> int:
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--legs", default="compare,c4,serve",
+                    help="extra legs after the headline: compare (other precision, per-field errors), c4, serve")
+    ap.add_argument("--c4-steps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-steps", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
     ap.add_argument("--copy-mib", type=int, default=256, help="size of the reference copy kernel run (HBM counter scale)")
     args = ap.parse_args()
@@ -134,6 +267,7 @@ def main() -> int:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the periodic block is single-GPU, multi-GPU halo exchange is design-only)", file=sys.stderr)
         return 2
+    legs = {x for x in args.legs.split(",") if x}
 
     import numpy as np
     cfdx = _import_pkg()
@@ -147,29 +281,15 @@ def main() -> int:
     precision = cfdx.MIXED if args.precision == "mixed" else cfdx.FP64
     block = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision)
     block.set_case(case)
-    del case
     print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
     ctx = block.ctx
-    block.step(args.warmup, mach, dt, prm)
-    ctx.sync()
     t0 = time.perf_counter()
-    ctx.record(0)
-    block.step(args.steps, mach, dt, prm)
-    ctx.record(1)
-    ctx.sync()
-    t1 = time.perf_counter()
-    wall_s = t1 - t0
-    dev_ms = ctx.elapsed_ms(0, 1)
+    dev_ms, tk = time_block(block, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps)
+    wall_s = time.perf_counter() - t0
     ms_per_step = dev_ms / args.steps
     value = n * args.steps / (dev_ms / 1e3)
-    print(f"bench: {args.steps} steps wall {wall_s:.3f}s device {dev_ms:.1f}ms", file=sys.stderr, flush=True)
-
-    # per-kernel-class device time (HIP events on the block's stream, separate pass)
-    block.set_timing(True)
-    block.step(args.profile_steps, mach, dt, prm)
-    tk = block.timing()
-    block.set_timing(False)
+    print(f"bench: {args.steps} steps device {dev_ms:.1f}ms ({wall_s:.2f}s wall incl. timing pass)", file=sys.stderr, flush=True)
     face_launches = 9 * args.profile_steps
     face_ms = tk["face_ms"] / face_launches
     face_bytes = face_bytes_per_node(args.precision)
@@ -225,6 +345,35 @@ def main() -> int:
         "state_finite": finite,
         "copy_kernel": {"bytes_moved": 2 * nb, "ms": copy_ms, "GBs": 2 * nb / (copy_ms / 1e3) / 1e9},
     }
+
+    if "compare" in legs:
+        # the other precision on the same initial state and step count: throughput and per-field error
+        other = cfdx.FP64 if precision == cfdx.MIXED else cfdx.MIXED
+        ob = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=other, ctx=ctx)
+        ob.set_case(case)
+        o_ms, o_tk = time_block(ob, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps)
+        oflow = ob.get_flow()
+        ob.close()
+        mixed_flow, fp64_flow = (oflow, flow) if other == cfdx.MIXED else (flow, oflow)
+        o_value = n * args.steps / (o_ms / 1e3)
+        line["mixed_vs_fp64"] = {
+            "other_precision": "mixed" if other == cfdx.MIXED else "fp64",
+            "other_value": o_value,
+            "mixed_over_fp64": (o_value / value) if other == cfdx.MIXED else (value / o_value),
+            "other_kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in o_tk.items() if k.endswith("_ms")},
+            "steps_compared": args.warmup + args.steps + args.profile_steps,
+            "field_errors": field_errors(mixed_flow, fp64_flow),
+        }
+        print(f"bench: other precision {o_value:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    del case
+    block.close()
+
+    if "c4" in legs:
+        line["batched_c4"] = batched_c4(cfdx, args.c4_steps, 1)
+        print(f"bench: c4 {line['batched_c4']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    if "serve" in legs:
+        line["serving"] = serving(cfdx, args.serve_requests, args.serve_rate, args.serve_steps)
+
     if not args.no_cpu:
         cb = cpu_baseline(cfdx)
         if cb:

@@ -47,6 +47,14 @@ class BlockDesc(C.Structure):
                 ("precision", C.c_int32)]
 
 
+INIT_KINDS = {"freestream": 0, "vortex": 1, "tgv": 2}
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [("length", C.c_double * 3), ("amp", C.c_double), ("phase", C.c_double * 3), ("init", C.c_int32),
+                ("pad_", C.c_int32), ("mach", C.c_double), ("beta", C.c_double), ("center", C.c_double * 2)]
+
+
 class CfdxError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"cfdx error {code}: {msg}")
@@ -72,6 +80,7 @@ EXPORTS = {
     "cfdx_ctx_stream": (_vp, [_vp]),
     "cfdx_sync": (C.c_int, [_vp]),
     "cfdx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "cfdx_mem_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     "cfdx_device_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "cfdx_device_free": (C.c_int, [_vp, _vp]),
     "cfdx_copy_to_device": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
@@ -88,6 +97,8 @@ EXPORTS = {
     "cfdx_block_set_geometry": (C.c_int, [_vp, _vp, _vp, _vp]),
     "cfdx_block_set_flow": (C.c_int, [_vp, _vp]),
     "cfdx_block_get_flow": (C.c_int, [_vp, _vp]),
+    "cfdx_block_init_case": (C.c_int, [_vp, C.POINTER(CaseDesc)]),
+    "cfdx_block_cfl_dt": (C.c_int, [_vp, _d, C.POINTER(_d)]),
     "cfdx_block_configure": (C.c_int, [_vp, _d, _d, C.POINTER(Params)]),
     "cfdx_block_step": (C.c_int, [_vp, _d, _d, C.c_int, C.POINTER(Params)]),
     "cfdx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
@@ -259,6 +270,12 @@ class Ctx:
         except Exception:
             pass
 
+    def mem_info(self) -> tuple[int, int]:
+        """(free, total) bytes of the device (hipMemGetInfo)."""
+        f, t = _i64(0), _i64(0)
+        _check(_lib.cfdx_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     def record(self, slot: int) -> None:
         _check(_lib.cfdx_event_record(self.h, slot))
 
@@ -393,6 +410,22 @@ class Block:
         deter = np.ascontiguousarray(case["deter"][1:])
         _check(_lib.cfdx_block_set_geometry(self.h, _ptr(det), _ptr(metric), _ptr(deter)))
         self.set_flow(case["FLOW"][1:])
+
+    def init_case(self, grid: str = "box", length=(10.0, 10.0, 10.0), amp: float = 0.0, phases=(0.0, 0.0, 0.0),
+                  init: str = "freestream", mach: float = 0.5, beta: float = 5.0, center=(5.0, 5.0)) -> None:
+        """Geometry (exact metrics of the map) and initial state built on the device; the same
+        case as periodic_case(..., metrics='analytic') without the host arrays. Asynchronous."""
+        if grid not in ("box", "sinusoid"):
+            raise ValueError(grid)
+        c = CaseDesc((C.c_double * 3)(*length), amp if grid == "sinusoid" else 0.0, (C.c_double * 3)(*phases),
+                     INIT_KINDS[init], 0, mach, beta, (C.c_double * 2)(*center))
+        _check(_lib.cfdx_block_init_case(self.h, C.byref(c)))
+
+    def cfl_dt(self, cfl: float) -> float:
+        """cfl / max spectral radius of the current state (synchronous)."""
+        dt = _d(0.0)
+        _check(_lib.cfdx_block_cfl_dt(self.h, cfl, C.byref(dt)))
+        return dt.value
 
     def set_flow(self, flow: np.ndarray) -> None:
         flow = np.ascontiguousarray(flow)

@@ -330,14 +330,13 @@ extern "C" int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow)
     CFDX_TRY(cfdx::enter(b->ctx));
     const long long n = b->B.n;
     std::vector<double> h((size_t)6 * n);
-    double *tmp = nullptr;
-    CFDX_HIP(hipMalloc((void **)&tmp, (size_t)5 * n * 8), "get_flow hipMalloc");
+    // the face-flux arrays are scratch between stages: 15 n flux-type values >= 5 n doubles
+    double *tmp = (double *)b->B.FH[0][0];
     cfdx::blk::prims(b->B, b->desc.precision, b->ctx->stream, tmp);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), tmp, (size_t)5 * n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data() + 5 * n, b->B.mu, (size_t)n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
-    (void)hipFree(tmp);
     if (e != hipSuccess) return cfdx::hip_fail(e, "get_flow");
     const double M = b->last_mach;
     for (long long g = 0; g < n; ++g) {
@@ -348,6 +347,41 @@ extern "C" int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow)
         f.t = (1.4 * M * M) * (f.p / f.rho);
         f.a = std::sqrt(1.4 * f.p / f.rho);
     }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c)
+{
+    if (!b || !c) return cfdx::fail(CFDX_E_INVALID, "init_case: null argument");
+    if (!(c->length[0] > 0 && c->length[1] > 0 && c->length[2] > 0) || !(c->mach > 0) || !std::isfinite(c->amp))
+        return cfdx::fail(CFDX_E_INVALID, "init_case: lengths and Mach must be positive");
+    if (c->init < CFDX_INIT_FREESTREAM || c->init > CFDX_INIT_TGV)
+        return cfdx::fail(CFDX_E_INVALID, "init_case: unknown initial state %d", c->init);
+    CFDX_TRY(cfdx::enter(b->ctx));
+    cfdx::blk::init_geometry(b->B, c->length, c->amp, c->phase, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("init_geometry"));
+    cfdx::blk::init_state(b->B, c->length, c->amp, c->phase, c->init, c->mach, c->beta, c->center[0], c->center[1], b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("init_state"));
+    cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("k_base"));
+    b->last_mach = c->mach;
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_cfl_dt(cfdx_block *b, double cfl, double *dt)
+{
+    if (!b || !dt || !(cfl > 0)) return cfdx::fail(CFDX_E_INVALID, "cfl_dt: null argument or cfl <= 0");
+    CFDX_TRY(cfdx::enter(b->ctx));
+    // per-workgroup maxima in the face-flux scratch (at most 1025 doubles)
+    double *part = (double *)b->B.FH[0][0];
+    const int np = cfdx::blk::cfl_parts(b->B);
+    cfdx::blk::cfl_radius(b->B, part, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("cfl_radius"));
+    double r = 0.0;
+    CFDX_HIP(hipMemcpyAsync(&r, part + np, sizeof(double), hipMemcpyDeviceToHost, b->ctx->stream), "cfl_dt copy");
+    CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "cfl_dt sync");
+    if (!(r > 0) || !std::isfinite(r)) return cfdx::fail(CFDX_E_NONFINITE, "cfl_dt: spectral radius %g", r);
+    *dt = cfl / r;
     return CFDX_OK;
 }
 

@@ -53,5 +53,12 @@ void base(const BlockDev &B, int precision, hipStream_t st);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
 uint32_t tiles_of(const BlockDev &B, int kind);
 
+// device-side set-up (init_kernels.hip)
+void init_geometry(const BlockDev &B, const double L[3], double amp, const double ph[3], hipStream_t st);
+void init_state(const BlockDev &B, const double L[3], double amp, const double ph[3], int kind, double mach, double beta,
+                double xc, double yc, hipStream_t st);
+int cfl_parts(const BlockDev &B);
+void cfl_radius(const BlockDev &B, double *part, hipStream_t st);
+
 }  // namespace blk
 }  // namespace cfdx

@@ -94,6 +94,17 @@ extern "C" int cfdx_device_count(int *count)
     return CFDX_OK;
 }
 
+extern "C" int cfdx_mem_info(cfdx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return cfdx::fail(CFDX_E_INVALID, "mem_info: null argument");
+    CFDX_TRY(cfdx::enter(ctx));
+    size_t f = 0, t = 0;
+    CFDX_HIP(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return CFDX_OK;
+}
+
 extern "C" int cfdx_device_alloc(cfdx_ctx *ctx, size_t bytes, void **dptr)
 {
     CFDX_TRY(cfdx::enter(ctx));

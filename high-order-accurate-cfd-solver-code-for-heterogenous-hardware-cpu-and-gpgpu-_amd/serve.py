@@ -1,0 +1,214 @@
+"""Serving loop: parametric flow solves admitted as they arrive, stepped together on one GPU.
+
+Each request is one block (grid size, map, initial state, Mach, Re, number of RK3 steps). The
+server keeps the state of every admitted request resident in HBM and steps all of them in the
+same batched launches (continuous batching): between chunks of steps it retires finished
+requests (result copied to the host, block freed) and admits queued ones while the device has
+free memory (hipMemGetInfo minus a reserve, optionally capped by a byte budget). A request's
+arithmetic does not depend on what else is in the batch, so its result is the same alone and
+in the mix (the tests check bitwise equality).
+
+Clocks: "wall" takes arrival times in seconds from the start of run(); "steps" measures arrival
+in RK3 steps executed by the server, which makes a schedule reproducible for tests.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import FP64, MIXED, Batch, Block, Ctx, default_params, euler_params
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    ni: int
+    nj: int
+    nk: int
+    nsteps: int
+    arrival: float = 0.0             # seconds ("wall") or steps ("steps")
+    grid: str = "sinusoid"
+    length: tuple = (2 * math.pi,) * 3
+    amp_frac: float = 0.05           # map amplitude as a fraction of the i spacing
+    phases: tuple = (0.0, 0.0, 0.0)
+    init: str = "tgv"
+    mach: float = 0.1
+    reyl: float = 1600.0
+    viscous: bool = True
+    beta: float = 5.0
+    center: tuple = (5.0, 5.0)
+    cfl: float = 0.5
+
+    @property
+    def nodes(self) -> int:
+        return self.ni * self.nj * self.nk
+
+    def params(self):
+        if not self.viscous:
+            return euler_params()
+        return default_params(mach=self.mach, reyl=self.reyl)
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    dt: float
+    flow: np.ndarray | None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def make_block(req: Request, precision: int, ctx: Ctx) -> tuple[Block, float]:
+    """Creates, initialises (on the device) and configures the block of one request; returns it
+    with its CFL time step (computed once from the initial state)."""
+    b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx)
+    amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
+    b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
+                beta=req.beta, center=req.center)
+    dt = b.cfl_dt(req.cfl)
+    b.configure(req.mach, dt, req.params())
+    return b, dt
+
+
+def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) -> Result:
+    """The request on its own (single-block steps), for equivalence checks."""
+    b, dt = make_block(req, precision, ctx)
+    b.step(req.nsteps, req.mach, dt, req.params())
+    flow = b.get_flow() if keep_flow else None
+    b.close()
+    return Result(req.rid, 0.0, 0.0, 0.0, dt, flow)
+
+
+class Server:
+    def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
+                 budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
+                 grid_cap: int = 0, batched: bool = True, keep_flow: bool = True):
+        self.ctx = ctx or Ctx()
+        self.precision = precision
+        self.reserve = reserve_bytes
+        self.budget = budget_bytes
+        self.max_active = max_active if batched else 1
+        self.chunk = max(1, chunk)
+        self.graph = graph
+        self.keep_flow = keep_flow
+        self.batch = Batch(precision, self.ctx)
+        if grid_cap:
+            self.batch.set_grid(grid_cap)
+        self.peak_active = 0
+        self.peak_bytes = 0
+        self.chunks = 0
+
+    def _fits(self, req: Request, used: int) -> bool:
+        need = Block.bytes_for(req.ni, req.nj, req.nk, self.precision)
+        if self.budget is not None and used + need > self.budget:
+            return False
+        free, _ = self.ctx.mem_info()
+        return free - need >= self.reserve
+
+    def run(self, requests: list[Request], clock: str = "wall") -> dict:
+        if clock not in ("wall", "steps"):
+            raise ValueError(clock)
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        queue: collections.deque[Request] = collections.deque()
+        active: dict[int, dict] = {}
+        results: dict[int, Result] = {}
+        used = 0
+        vclock = 0  # steps executed (steps clock)
+        t0 = time.perf_counter()
+
+        def now() -> float:
+            return float(vclock) if clock == "steps" else time.perf_counter() - t0
+
+        while pending or queue or active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                queue.append(pending.popleft())
+            # admission: FIFO while memory and slots allow
+            while queue and len(active) < self.max_active and self._fits(queue[0], used):
+                req = queue.popleft()
+                b, dt = make_block(req, self.precision, self.ctx)
+                self.batch.add(b)
+                nb = Block.bytes_for(req.ni, req.nj, req.nk, self.precision)
+                used += nb
+                active[req.rid] = dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb)
+            if queue and not active:
+                raise RuntimeError(f"request {queue[0].rid} ({queue[0].ni}x{queue[0].nj}x{queue[0].nk}) does not fit "
+                                   "in the free device memory")
+            if not active:
+                if pending:  # idle until the next arrival
+                    if clock == "steps":
+                        vclock = max(vclock, int(math.ceil(pending[0].arrival)))
+                    else:
+                        time.sleep(max(0.0, pending[0].arrival - now()))
+                continue
+            self.peak_active = max(self.peak_active, len(active))
+            self.peak_bytes = max(self.peak_bytes, used)
+            k = min(self.chunk, min(a["left"] for a in active.values()))
+            if clock == "steps" and pending:
+                k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
+            self.batch.step(k, self.graph)
+            self.ctx.sync()
+            self.chunks += 1
+            vclock += k
+            done = []
+            for rid, a in active.items():
+                a["left"] -= k
+                if a["left"] == 0:
+                    done.append(rid)
+            for rid in done:
+                a = active.pop(rid)
+                b = a["block"]
+                flow = b.get_flow() if self.keep_flow else None
+                if not self.keep_flow:
+                    self.ctx.sync()
+                self.batch.remove(b)
+                b.close()
+                used -= a["bytes"]
+                results[rid] = Result(rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow)
+        return results
+
+    def close(self) -> None:
+        self.batch.close()
+
+
+def report(requests: list[Request], results: dict[int, Result]) -> dict:
+    """p50 / p99 latency (arrival -> result on the host) and aggregate cell-updates/s."""
+    lat = np.array([results[r.rid].latency for r in requests])
+    cells = float(sum(r.nodes * r.nsteps for r in requests))
+    span = max(res.finished for res in results.values()) - min(r.arrival for r in requests)
+    return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max()), "makespan_s": span,
+            "cell_updates": cells, "cell_updates_per_s": cells / span if span > 0 else float("nan")}
+
+
+def periodic_stream(n: int, rate: float, seed: int = 20261202, sizes=((128, 64, 16), (192, 96, 24), (256, 128, 32),
+                                                                       (384, 128, 32)),
+                    nsteps: int = 300) -> list[Request]:
+    """Poisson arrivals (rate per second) of periodic TGV / vortex solves with seeded sizes,
+    Mach U(0.1, 0.6) and Re U(500, 5000); the same request stream every call for a given seed."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        ni, nj, nk = sizes[rng.integers(len(sizes))]
+        tgv = bool(rng.integers(2))
+        mach = float(rng.uniform(0.1, 0.6))
+        reyl = float(rng.uniform(500.0, 5000.0))
+        phases = tuple(float(x) for x in rng.uniform(0, 2 * math.pi, 3))
+        if tgv:
+            out.append(Request(rid, ni, nj, nk, nsteps, t, grid="sinusoid", length=(2 * math.pi,) * 3, phases=phases,
+                               init="tgv", mach=mach, reyl=reyl))
+        else:
+            out.append(Request(rid, ni, nj, nk, nsteps, t, grid="sinusoid", length=(10.0,) * 3, phases=phases,
+                               init="vortex", mach=mach, reyl=reyl, center=(5.0, 5.0)))
+    return out

@@ -98,6 +98,9 @@ int cfdx_sync(cfdx_ctx *ctx);
 /* Number of HIP devices visible to this process. */
 int cfdx_device_count(int *count);
 
+/* Free and total device memory of the context's device (hipMemGetInfo); used to admit work. */
+int cfdx_mem_info(cfdx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
 /* Device memory and copies on the context's device (the drop-in kernels take
  * device pointers; callers may equally use their own hipMalloc'd buffers).
  * Copies are ordered on the context stream and complete before returning. */
@@ -157,6 +160,27 @@ int cfdx_block_destroy(cfdx_block *b);
 int cfdx_block_set_geometry(cfdx_block *b, const double *det, const cfdx_metric *metric, const cfdx_deter *deter);
 int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow);
 int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow);
+
+/* Device-side set-up of a periodic block (no host arrays): geometry of the seeded sinusoidal map
+ * x = s1 + amp sin(2pi s2/L2 + ph0) sin(2pi s3/L3 + ph0) (and cyclic; amp = 0 gives the Cartesian
+ * box), exact Jacobian -> det and node metrics, half-point metrics, and the initial state.
+ * Asynchronous on the context stream. */
+enum { CFDX_INIT_FREESTREAM = 0, CFDX_INIT_VORTEX = 1, CFDX_INIT_TGV = 2 };
+typedef struct {
+    double length[3];   /* period along i, j, k */
+    double amp;         /* map amplitude */
+    double phase[3];    /* map phases */
+    int32_t init;       /* CFDX_INIT_* */
+    int32_t pad_;
+    double mach;        /* free-stream Mach (p_inf = 1 / (1.4 M^2)) */
+    double beta;        /* vortex strength */
+    double center[2];   /* vortex centre (x, y) */
+} cfdx_case_desc;
+int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c);
+
+/* dt = cfl / max over nodes of sum_d (|u . grad xi_d| + c |grad xi_d|) of the current state.
+ * Synchronous. */
+int cfdx_block_cfl_dt(cfdx_block *b, double cfl, double *dt);
 
 /* Run parameters of the block (Weno-argument Mach for t = gamma M^2 p/rho, fixed del_t, physical
  * constants, viscous switch), used by cfdx_block_step and by batched steps. */

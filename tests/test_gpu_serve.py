@@ -1,0 +1,108 @@
+"""Device-side case set-up and the serving loop (GPU).
+
+- a block initialised on the device (geometry of the map, half-point metrics, initial state)
+  gives the same right-hand side as the same case built on the host and uploaded;
+- the device CFL time step equals the host formula;
+- every request served in a mix of uneven, staggered arrivals with a memory budget that forces
+  queueing gives bitwise the result it gets alone (plain FP64 and mixed).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cfdx
+from cfdx import serve
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if cfdx.device_count() < 1:
+        pytest.skip("no HIP device")
+    return cfdx.Ctx(0)
+
+
+CASES = [
+    dict(ni=16, nj=12, nk=10, grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * 2 * math.pi / 16,
+         phases=(0.3, 1.1, 2.0), init="tgv", mach=0.1),
+    dict(ni=12, nj=14, nk=8, grid="box", length=(10.0,) * 3, init="vortex", mach=0.5, beta=5.0, center=(5.0, 5.0)),
+    dict(ni=10, nj=10, nk=12, grid="sinusoid", length=(10.0, 10.0, 10.0), amp=0.2, phases=(2.0, 0.5, 1.0),
+         init="vortex", mach=0.3, beta=5.0, center=(4.0, 6.0)),
+]
+
+
+@pytest.mark.parametrize("kw", CASES, ids=["tgv-sinusoid", "vortex-box", "vortex-sinusoid"])
+def test_device_init_matches_host(ctx, kw):
+    ni, nj, nk = kw["ni"], kw["nj"], kw["nk"]
+    hk = {k: v for k, v in kw.items() if k not in ("ni", "nj", "nk")}
+    case = cfdx.periodic_case(ni, nj, nk, **hk)
+    prm = cfdx.default_params(mach=kw["mach"], reyl=1600.0)
+    host = cfdx.Block(ni, nj, nk, ctx=ctx)
+    host.set_case(case)
+    dev = cfdx.Block(ni, nj, nk, ctx=ctx)
+    dev.init_case(**hk)
+    fh, fd = host.get_flow(), dev.get_flow()
+    for f in FIELDS:
+        scale = max(np.abs(fh[f]).max(), 1e-300)
+        assert np.abs(fd[f] - fh[f]).max() <= 1e-14 * scale, f
+    # the inputs differ in the last bits of sin / cos (device vs host libm). The RHS is a difference
+    # of face fluxes up to ~(E + p)|u| ~ 1e3 |L| at Mach 0.1, so an input difference of a few ulp
+    # shows up at ~1e-12 of max |L|; the bound is 1e-11 of the largest RHS component.
+    Lh, Ld = host.rhs(prm), dev.rhs(prm)
+    scale = np.abs(Lh).max()
+    for c in range(5):
+        err = np.abs(Ld[c] - Lh[c]).max() / scale
+        assert err <= 1e-11, (c, err)
+    dt_host = cfdx.cfl_dt(case, 0.5)
+    dt_dev = dev.cfl_dt(0.5)
+    assert abs(dt_dev - dt_host) <= 1e-13 * dt_host, (dt_dev, dt_host)
+
+
+def small_stream():
+    sizes = [(16, 12, 10), (12, 12, 12), (20, 10, 8), (10, 14, 12), (16, 16, 8), (12, 10, 14)]
+    reqs = []
+    for rid, (ni, nj, nk) in enumerate(sizes):
+        tgv = rid % 2 == 0
+        reqs.append(serve.Request(rid, ni, nj, nk, nsteps=6 + 2 * rid, arrival=3 * rid,
+                                  grid="sinusoid", length=(2 * math.pi,) * 3 if tgv else (10.0,) * 3,
+                                  phases=(0.1 * rid, 0.7, 1.3), init="tgv" if tgv else "vortex",
+                                  mach=0.1 + 0.08 * rid, reyl=500.0 + 700.0 * rid, viscous=rid != 3))
+    return reqs
+
+
+@pytest.mark.parametrize("precision", [cfdx.FP64, cfdx.MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_serving_alone_vs_mix(ctx, precision, graph):
+    reqs = small_stream()
+    # a budget of three of the largest blocks forces queueing and admission as slots free up
+    big = max(cfdx.Block.bytes_for(r.ni, r.nj, r.nk, precision) for r in reqs)
+    srv = serve.Server(precision=precision, ctx=ctx, reserve_bytes=0, budget_bytes=3 * big, chunk=4, graph=graph)
+    res = srv.run(reqs, clock="steps")
+    srv.close()
+    assert sorted(res) == [r.rid for r in reqs]
+    assert 2 <= srv.peak_active <= 3
+    for r in reqs:
+        alone = serve.run_alone(r, precision, ctx)
+        assert res[r.rid].dt == alone.dt
+        for f in FIELDS:
+            assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
+        assert all(np.isfinite(res[r.rid].flow[f]).all() for f in FIELDS)
+    rep = serve.report(reqs, res)
+    assert rep["p99_s"] >= rep["p50_s"] > 0
+
+
+def test_serving_wall_clock(ctx):
+    reqs = small_stream()
+    for r in reqs:
+        r.arrival = 0.01 * r.rid
+    srv = serve.Server(precision=cfdx.MIXED, ctx=ctx, reserve_bytes=0, chunk=3)
+    res = srv.run(reqs, clock="wall")
+    srv.close()
+    rep = serve.report(reqs, res)
+    print("serving (wall clock):", rep)
+    assert rep["requests"] == len(reqs) and rep["cell_updates_per_s"] > 0
+    assert all(res[r.rid].finished >= res[r.rid].admitted >= r.arrival - 1e-9 for r in reqs)
