@@ -22,16 +22,21 @@ namespace cfdx {
 namespace blk {
 
 // Loads and stores at a 32-bit element index from a uniform base pointer, so the compiler
-// emits SGPR-base + VGPR-offset global accesses (a block holds < 2^29 nodes).
+// emits SGPR-base + VGPR-offset accesses (a block holds < 2^29 nodes). The array pointers come
+// from a descriptor in memory, so they are cast to the global address space explicitly: global
+// (not flat) instructions count only on the vector-memory wait counter, not on the LDS one.
+#define CFDX_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ __forceinline__ T ld(const T *base, uint32_t i)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
+    const CFDX_GLOBAL char *p = (const CFDX_GLOBAL char *)base;
+    return *(const CFDX_GLOBAL T *)(p + i * (uint32_t)sizeof(T));
 }
 template <typename T>
 __device__ __forceinline__ void st(T *base, uint32_t i, T v)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
+    CFDX_GLOBAL char *p = (CFDX_GLOBAL char *)base;
+    *(CFDX_GLOBAL T *)(p + i * (uint32_t)sizeof(T)) = v;
 }
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }

@@ -23,6 +23,50 @@ template <typename T> __device__ __forceinline__ T div_(T a, T b);
 template <> __device__ __forceinline__ double div_<double>(double a, double b) { return a / b; }
 template <> __device__ __forceinline__ float div_<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// Reciprocal, square root and reciprocal square root of a strictly positive normal operand (a
+// density, a squared norm, a sum of positive terms), to about one ulp: the hardware estimate
+// refined by Newton-Raphson (reciprocal) and Goldschmidt (square root) steps in double, the
+// hardware instructions in float. Not for 0, inf or NaN operands (no range scaling).
+template <typename T> __device__ __forceinline__ T pinv_(T x);
+template <> __device__ __forceinline__ double pinv_<double>(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+template <> __device__ __forceinline__ float pinv_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// s = sqrt(x) and h = 1 / (2 sqrt(x)) by one Goldschmidt step on the rsq estimate, then two
+// residual corrections of s
+__device__ __forceinline__ double gold_sqrt(double x, double &h)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+template <typename T> __device__ __forceinline__ T psqrt_(T x);
+template <> __device__ __forceinline__ double psqrt_<double>(double x) { double h; return gold_sqrt(x, h); }
+template <> __device__ __forceinline__ float psqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+// 1 / sqrt(x)
+template <typename T> __device__ __forceinline__ T prsq_(T x);
+template <> __device__ __forceinline__ double prsq_<double>(double x)
+{
+    double h;
+    const double s = gold_sqrt(x, h);
+    const double r = 2.0 * h;
+    return __builtin_fma(r, __builtin_fma(-s, r, 1.0), r);   // one Newton step on 1 / s
+}
+template <> __device__ __forceinline__ float prsq_<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // Conserved state Q = (rho, rho u, rho v, rho w, p/0.4 + rho |u|^2 / 2) of primitives (U_Q.cl:118-122).
 template <typename T>
 __device__ __forceinline__ void prim_to_q(T rho, T u, T v, T w, T p, T (&q)[5])
@@ -39,15 +83,15 @@ __device__ __forceinline__ void prim_to_q(T rho, T u, T v, T w, T p, T (&q)[5])
 template <typename T>
 __device__ __forceinline__ T enthalpy(T rho, T u, T v, T w, T p)
 {
-    return div_(p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w), rho);   // (p/0.4 + ke + p) / rho
+    return (p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w)) * pinv_(rho);   // (p/0.4 + ke + p) / rho
 }
 
 // Roe average of the face between states a and b (U_Q.cl:60-109): (u, v, w, h, rho).
 template <typename T>
 __device__ __forceinline__ void roe_average(T ra, T ua, T va, T wa, T ha, T rb, T ub, T vb, T wb, T hb, T (&r)[5])
 {
-    const T sa = sqrt_(ra), sb = sqrt_(rb);
-    const T is = inv_(sa + sb);
+    const T sa = psqrt_(ra), sb = psqrt_(rb);
+    const T is = pinv_(sa + sb);
     r[0] = (sb * ub + sa * ua) * is;
     r[1] = (sb * vb + sa * va) * is;
     r[2] = (sb * wb + sa * wa) * is;
@@ -119,7 +163,7 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
             nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
             dn_ = w0 + w1 + w2;
         }
-        const T r = inv_(dp_ * dn_);
+        const T r = pinv_(dp_ * dn_);
         pos = np_ * (dn_ * r);
         neg = nn_ * (dp_ * r);
     } else {
@@ -145,14 +189,14 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
                                                       T (&qp)[5], T (&qn)[5])
 {
     const T u = roe[0], v = roe[1], w = roe[2], h = roe[3], rho = roe[4];
-    const T ikk = inv_(sqrt_(k0 * k0 + k1 * k1 + k2 * k2));
+    const T ikk = prsq_(k0 * k0 + k1 * k1 + k2 * k2);
     const T kx = k0 * ikk, ky = k1 * ikk, kz = k2 * ikk;
     const T q2 = u * u + v * v + w * w;
-    const T a = sqrt_(T(0.4) * (h - T(0.5) * q2));
+    const T a = psqrt_(T(0.4) * (h - T(0.5) * q2));
     const T theta = kx * u + ky * v + kz * w;
     const T phi = T(0.2) * q2;
-    const T irho = inv_(rho);
-    const T ia = inv_(a);
+    const T irho = pinv_(rho);
+    const T ia = pinv_(a);
     const T iaa = ia * ia;
     const T s2 = T(0.70710678118654752440);           // 1/sqrt(2)
     const T alpha = rho * ia * s2;                      // rho / (sqrt2 a)
@@ -203,7 +247,7 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
 template <typename T>
 __device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k2, T kk2, T detf, T (&c)[5], T &lam)
 {
-    const T r = inv_(Q[0]);
+    const T r = pinv_(Q[0]);
     const T ke = (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * r;
     const T P = T(0.4) * (Q[4] - T(0.5) * ke);
     const T Un = (k0 * Q[1] + k1 * Q[2] + k2 * Q[3]) * r;
@@ -212,7 +256,7 @@ __device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k
     c[2] = detf * (Q[2] * Un + k1 * P);
     c[3] = detf * (Q[3] * Un + k2 * P);
     c[4] = detf * ((T(1.4) * Q[4] - T(0.2) * ke) * Un);
-    lam = Un + sqrt_(T(1.4) * P * r * kk2);
+    lam = Un + psqrt_(T(1.4) * P * r * kk2);
 }
 
 // LLF face flux with alpha = max(|lambda_5|) of the two interface states (Weno.cl:2859-2947).
