@@ -125,6 +125,18 @@ def cpu_baseline(cfdx, max_seconds: float = 15.0):
             "seconds": round(t3 - t2, 2)}
 
 
+def tuned_tiles(cfdx, cdef, precision, mach, prm):
+    """Tile variants for this grid and precision, tuned on a scratch block of the same case set up
+    on the device (the tuner runs real steps), or taken from the tile cache."""
+    from cfdx import tune
+    scratch = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision)
+    scratch.init_case(**cdef["kw"])
+    dt = scratch.cfl_dt(0.5)
+    res = tune.tune_block(scratch, mach, dt, prm, steps=2, cache=tune.TileCache())
+    scratch.close()
+    return res
+
+
 def time_block(block, ctx, mach, dt, prm, steps, warmup, profile_steps):
     """Warm-up, K timed steps bracketed by HIP events on the block's stream, then a separate
     per-kernel-class timing pass."""
@@ -192,6 +204,8 @@ def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int
     q = cfdx.Batch(cfdx.MIXED, ctx)
     for b, *_ in blocks:
         q.add(b)
+    from cfdx import tune
+    tuned = tune.tune_batch(q, ctx, key=f"batch:c4:{seed}:{nblocks}:p{cfdx.MIXED}", steps=1, cache=tune.TileCache())
     if grid_cap:
         q.set_grid(grid_cap)
     q.step(warmup, True)
@@ -221,6 +235,7 @@ def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int
     return {"workload": "64 periodic cubes, edge U{32,48,64,96,128}, vortex/TGV, Mach U(0.1,0.6), Re U(500,5000)",
             "edges": hist, "nodes": nodes, "steps": steps, "precision": "mixed", "graph": True,
             "value": nodes * steps / (ms / 1e3), "unit": "cell-updates/s", "ms_per_step": ms / steps,
+            "tiles": {"face_i_j_k_node": tuned["tiles"], "grid_cap": tuned["grid_cap"], "cached": tuned["cached"]},
             "plain_fp64_sequential": nodes * steps / (ms_plain / 1e3),
             "speedup_vs_plain_fp64": ms_plain / ms, "state_finite": finite}
 
@@ -257,6 +272,7 @@ def main() -> int:
     ap.add_argument("--legs", default="compare,c4,serve",
                     help="extra legs after the headline: compare (other precision, per-field errors), c4, serve")
     ap.add_argument("--c4-steps", type=int, default=5)
+    ap.add_argument("--no-tune", dest="tune", action="store_false", help="default tiles instead of the autotuned ones")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-steps", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -279,8 +295,11 @@ def main() -> int:
     dt = cfdx.cfl_dt(case, 0.5)
     n = case["n"]
     precision = cfdx.MIXED if args.precision == "mixed" else cfdx.FP64
+    tiles = tuned_tiles(cfdx, cdef, precision, mach, prm) if args.tune else None
     block = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision)
     block.set_case(case)
+    if tiles:
+        block.set_tiles(tuple(tiles["tiles"]))
     print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
     ctx = block.ctx
@@ -344,13 +363,18 @@ def main() -> int:
         "kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in tk.items() if k.endswith("_ms")},
         "state_finite": finite,
         "copy_kernel": {"bytes_moved": 2 * nb, "ms": copy_ms, "GBs": 2 * nb / (copy_ms / 1e3) / 1e9},
+        "tiles": ({"face_i_j_k_node": tiles["tiles"], "cached": tiles["cached"], "times_ms": tiles.get("times_ms")}
+                  if tiles else None),
     }
 
     if "compare" in legs:
         # the other precision on the same initial state and step count: throughput and per-field error
         other = cfdx.FP64 if precision == cfdx.MIXED else cfdx.MIXED
+        otiles = tuned_tiles(cfdx, cdef, other, mach, prm) if args.tune else None
         ob = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=other, ctx=ctx)
         ob.set_case(case)
+        if otiles:
+            ob.set_tiles(tuple(otiles["tiles"]))
         o_ms, o_tk = time_block(ob, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps)
         oflow = ob.get_flow()
         ob.close()

@@ -55,6 +55,17 @@ class CaseDesc(C.Structure):
                 ("pad_", C.c_int32), ("mach", C.c_double), ("beta", C.c_double), ("center", C.c_double * 2)]
 
 
+class Tiles(C.Structure):
+    _fields_ = [("face", C.c_int32 * 3), ("node", C.c_int32)]
+
+    def as_tuple(self) -> tuple:
+        return (self.face[0], self.face[1], self.face[2], self.node)
+
+    @classmethod
+    def of(cls, t) -> "Tiles":
+        return cls((C.c_int32 * 3)(t[0], t[1], t[2]), t[3])
+
+
 class CfdxError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"cfdx error {code}: {msg}")
@@ -111,6 +122,11 @@ EXPORTS = {
     "cfdx_block_rhs": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "cfdx_block_set_timing": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_timing": (C.c_int, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(C.c_int)]),
+    "cfdx_block_timing_split": (C.c_int, [_vp, C.POINTER(_d)]),
+    "cfdx_tile_variants": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "cfdx_block_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
+    "cfdx_block_get_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
+    "cfdx_batch_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_geom_periodic_table": (C.c_int, [_i32, _i32, _i32, _vp, _vp]),
     "cfdx_geom_coords_box": (C.c_int, [_i32, _i32, _i32, _vp, _vp]),
     "cfdx_geom_coords_sinusoid": (C.c_int, [_i32, _i32, _i32, _vp, _d, _vp, _vp]),
@@ -141,6 +157,17 @@ def _ptr(a) -> C.c_void_p:
     if isinstance(a, np.ndarray):
         return C.c_void_p(a.ctypes.data)
     return C.c_void_p(a.ptr)
+
+
+def tile_variants(kind: int) -> list[tuple[int, int]]:
+    """Compiled tile shapes of a kernel class: 0 = face sweep along i (lines across j, faces
+    along i), 1 = face sweeps along j / k (lines across i, faces along the sweep), 2 = node
+    kernels (along i, along j)."""
+    n = C.c_int32(0)
+    _check(_lib.cfdx_tile_variants(kind, C.byref(n), None))
+    shapes = (C.c_int32 * (2 * n.value))()
+    _check(_lib.cfdx_tile_variants(kind, C.byref(n), shapes))
+    return [(shapes[2 * v], shapes[2 * v + 1]) for v in range(n.value)]
 
 
 def device_count() -> int:
@@ -451,6 +478,21 @@ class Block:
     def set_timing(self, on: bool) -> None:
         _check(_lib.cfdx_block_set_timing(self.h, int(on)))
 
+    def timing_split(self) -> list[float]:
+        """ms of the timed steps: viscous, face i, face j, face k, update."""
+        ms = (_d * 5)()
+        _check(_lib.cfdx_block_timing_split(self.h, ms))
+        return list(ms)
+
+    def set_tiles(self, t) -> None:
+        """t = (face_i, face_j, face_k, node) variant indices."""
+        _check(_lib.cfdx_block_set_tiles(self.h, C.byref(Tiles.of(t))))
+
+    def get_tiles(self) -> tuple:
+        t = Tiles()
+        _check(_lib.cfdx_block_get_tiles(self.h, C.byref(t)))
+        return t.as_tuple()
+
     def timing(self) -> dict:
         a, b, c, n = _d(), _d(), _d(), C.c_int()
         _check(_lib.cfdx_block_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(n)))
@@ -483,6 +525,9 @@ class Batch:
 
     def step(self, nsteps: int, graph: bool = True) -> None:
         _check(_lib.cfdx_batch_step(self.h, nsteps, int(graph)))
+
+    def set_tiles(self, t) -> None:
+        _check(_lib.cfdx_batch_set_tiles(self.h, C.byref(Tiles.of(t))))
 
     def close(self) -> None:
         if self.h:

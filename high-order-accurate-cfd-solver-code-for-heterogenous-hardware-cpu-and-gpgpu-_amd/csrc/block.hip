@@ -26,9 +26,10 @@ struct cfdx_block {
     bool configured = false;
     double last_mach = 0.5;
     bool timing = false;
-    hipEvent_t ev[4] = {};
-    double t_visc = 0, t_face = 0, t_update = 0;
+    hipEvent_t ev[6] = {};
+    double t_visc = 0, t_face[3] = {0, 0, 0}, t_update = 0;
     int launches = 0;
+    cfdx::blk::TileCfg tiles;
 };
 
 struct cfdx_batch {
@@ -43,6 +44,7 @@ struct cfdx_batch {
     LaunchList L;
     bool any_visc = false;
     uint32_t grid_cap = 0;
+    cfdx::blk::TileCfg tiles;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -89,7 +91,7 @@ int upload_single(cfdx_block *b)
     for (int kind = 0; kind < 4; ++kind) {
         B.tile0[kind] = 0;
         pre[2 * kind] = 0;
-        pre[2 * kind + 1] = cfdx::blk::tiles_of(b->B, kind);
+        pre[2 * kind + 1] = cfdx::blk::tiles_of(b->B, kind, b->tiles);
     }
     CFDX_HIP(hipMemcpyAsync(b->dB, &B, sizeof B, hipMemcpyHostToDevice, b->ctx->stream), "block descriptor upload");
     CFDX_HIP(hipMemcpyAsync(b->dprefix, pre, sizeof pre, hipMemcpyHostToDevice, b->ctx->stream), "block prefix upload");
@@ -103,9 +105,10 @@ LaunchList single_list(cfdx_block *b)
     LaunchList L;
     L.blks = b->dB;
     L.nblk = 1;
+    L.tiles = b->tiles;
     for (int kind = 0; kind < 4; ++kind) {
         L.prefix[kind] = b->dprefix + 2 * kind;
-        L.total[kind] = cfdx::blk::tiles_of(b->B, kind);
+        L.total[kind] = cfdx::blk::tiles_of(b->B, kind, b->tiles);
     }
     return L;
 }
@@ -117,12 +120,12 @@ int run_stage(cfdx_block *b, int mode)
     b->launches += (b->B.viscous ? 5 : 4);
     CFDX_TRY(cfdx::check_launch("block stage"));
     if (b->timing) {
-        float a = 0, c = 0, e = 0;
-        CFDX_HIP(hipEventSynchronize(b->ev[3]), "hipEventSynchronize");
-        (void)hipEventElapsedTime(&a, b->ev[0], b->ev[1]);
-        (void)hipEventElapsedTime(&c, b->ev[1], b->ev[2]);
-        (void)hipEventElapsedTime(&e, b->ev[2], b->ev[3]);
-        b->t_visc += a; b->t_face += c; b->t_update += e;
+        float t[5] = {0, 0, 0, 0, 0};
+        CFDX_HIP(hipEventSynchronize(b->ev[5]), "hipEventSynchronize");
+        for (int q = 0; q < 5; ++q) (void)hipEventElapsedTime(&t[q], b->ev[q], b->ev[q + 1]);
+        b->t_visc += t[0];
+        for (int d = 0; d < 3; ++d) b->t_face[d] += t[1 + d];
+        b->t_update += t[4];
     }
     return CFDX_OK;
 }
@@ -161,7 +164,7 @@ int upload_batch(cfdx_batch *q)
         for (int kind = 0; kind < 4; ++kind) {
             h[i].tile0[kind] = tot[kind];
             pre[kind * (q->cap + 1) + i] = tot[kind];
-            tot[kind] += cfdx::blk::tiles_of(q->blocks[i]->B, kind);
+            tot[kind] += cfdx::blk::tiles_of(q->blocks[i]->B, kind, q->tiles);
         }
         q->any_visc |= h[i].viscous != 0;
     }
@@ -176,6 +179,7 @@ int upload_batch(cfdx_batch *q)
         q->L.total[kind] = tot[kind];
     }
     q->L.grid_cap = q->grid_cap;
+    q->L.tiles = q->tiles;
     q->seen.resize(nb);
     for (size_t i = 0; i < nb; ++i) q->seen[i] = q->blocks[i]->version;
     q->dirty = false;
@@ -401,7 +405,8 @@ extern "C" int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nst
     if (nsteps < 0) return cfdx::fail(CFDX_E_INVALID, "block_step: nsteps < 0");
     configure(b, Mach, del_t, params);
     CFDX_TRY(upload_single(b));
-    b->t_visc = b->t_face = b->t_update = 0;
+    b->t_visc = b->t_update = 0;
+    b->t_face[0] = b->t_face[1] = b->t_face[2] = 0;
     b->launches = 0;
     for (int s = 0; s < nsteps; ++s)
         for (int j = 0; j < 3; ++j) CFDX_TRY(run_stage(b, j));
@@ -443,9 +448,73 @@ extern "C" int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face
 {
     if (!b) return cfdx::fail(CFDX_E_INVALID, "timing: null block");
     if (ms_visc) *ms_visc = b->t_visc;
-    if (ms_face) *ms_face = b->t_face;
+    if (ms_face) *ms_face = b->t_face[0] + b->t_face[1] + b->t_face[2];
     if (ms_update) *ms_update = b->t_update;
     if (launches) *launches = b->launches;
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_timing_split(cfdx_block *b, double ms[5])
+{
+    if (!b || !ms) return cfdx::fail(CFDX_E_INVALID, "timing_split: null argument");
+    ms[0] = b->t_visc;
+    for (int d = 0; d < 3; ++d) ms[1 + d] = b->t_face[d];
+    ms[4] = b->t_update;
+    return CFDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tile variants
+// ---------------------------------------------------------------------------
+namespace {
+int to_cfg(const cfdx_tiles *t, cfdx::blk::TileCfg &c)
+{
+    if (!t) return cfdx::fail(CFDX_E_INVALID, "tiles: null argument");
+    if (t->face[0] < 0 || t->face[0] >= cfdx::blk::N_FACE_I) return cfdx::fail(CFDX_E_INVALID, "tiles: face[0] variant %d out of range", t->face[0]);
+    for (int d = 1; d < 3; ++d)
+        if (t->face[d] < 0 || t->face[d] >= cfdx::blk::N_FACE_JK) return cfdx::fail(CFDX_E_INVALID, "tiles: face[%d] variant %d out of range", d, t->face[d]);
+    if (t->node < 0 || t->node >= cfdx::blk::N_NODE) return cfdx::fail(CFDX_E_INVALID, "tiles: node variant %d out of range", t->node);
+    for (int d = 0; d < 3; ++d) c.face[d] = t->face[d];
+    c.node = t->node;
+    return CFDX_OK;
+}
+}  // namespace
+
+extern "C" int cfdx_tile_variants(int kind, int32_t *count, int32_t *shapes)
+{
+    if (!count) return cfdx::fail(CFDX_E_INVALID, "tile_variants: null count");
+    const int(*tab)[2] = kind == 0 ? cfdx::blk::FACE_I_TILES : kind == 1 ? cfdx::blk::FACE_JK_TILES : cfdx::blk::NODE_TILES;
+    const int n = kind == 0 ? cfdx::blk::N_FACE_I : kind == 1 ? cfdx::blk::N_FACE_JK : cfdx::blk::N_NODE;
+    if (kind < 0 || kind > 2) return cfdx::fail(CFDX_E_INVALID, "tile_variants: kind %d is not 0 (face i), 1 (face j/k) or 2 (node)", kind);
+    *count = n;
+    if (shapes)
+        for (int v = 0; v < n; ++v) { shapes[2 * v] = tab[v][0]; shapes[2 * v + 1] = tab[v][1]; }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_set_tiles(cfdx_block *b, const cfdx_tiles *t)
+{
+    if (!b) return cfdx::fail(CFDX_E_INVALID, "block_set_tiles: null block");
+    cfdx::blk::TileCfg c;
+    CFDX_TRY(to_cfg(t, c));
+    if (!(c == b->tiles)) { b->tiles = c; ++b->version; }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_get_tiles(cfdx_block *b, cfdx_tiles *t)
+{
+    if (!b || !t) return cfdx::fail(CFDX_E_INVALID, "block_get_tiles: null argument");
+    for (int d = 0; d < 3; ++d) t->face[d] = b->tiles.face[d];
+    t->node = b->tiles.node;
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_batch_set_tiles(cfdx_batch *q, const cfdx_tiles *t)
+{
+    if (!q) return cfdx::fail(CFDX_E_INVALID, "batch_set_tiles: null batch");
+    cfdx::blk::TileCfg c;
+    CFDX_TRY(to_cfg(t, c));
+    if (!(c == q->tiles)) { q->tiles = c; q->dirty = true; }
     return CFDX_OK;
 }
 

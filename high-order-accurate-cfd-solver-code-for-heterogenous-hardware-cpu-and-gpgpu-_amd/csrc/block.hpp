@@ -10,11 +10,24 @@
 namespace cfdx {
 namespace blk {
 
-constexpr int NODE_TX = 64, NODE_TY = 4;   // node kernels: 64 x 4 tile at one k
+// Tile shapes compiled into the kernels; a launch list picks one of each at run time (the
+// autotuner times them on the actual grids). All tiles have 256 threads.
+//   face sweep along i: (TA lines across = j, TB faces along i)
+//   face sweeps along j and k: (TA lines across = i, TB faces along j or k)
+//   node kernels: (TX along i, TY along j) at one k
+constexpr int N_FACE_I = 3, N_FACE_JK = 4, N_NODE = 4;
+constexpr int FACE_I_TILES[N_FACE_I][2] = {{4, 64}, {2, 128}, {8, 32}};
+constexpr int FACE_JK_TILES[N_FACE_JK][2] = {{16, 16}, {32, 8}, {64, 4}, {8, 32}};
+constexpr int NODE_TILES[N_NODE][2] = {{64, 4}, {32, 8}, {128, 2}, {256, 1}};
 
-// face tiles: TB faces along the sweep direction times TA lines across it
-template <int DIR> struct FaceTile { static constexpr int TA = 16, TB = 16; };   // across = x, along = y or z
-template <> struct FaceTile<0> { static constexpr int TA = 4, TB = 64; };      // along = x, across = y
+struct TileCfg {
+    int face[3] = {0, 0, 0};   // variant of the i, j, k face sweeps
+    int node = 0;              // variant of the node kernels
+    bool operator==(const TileCfg &o) const
+    {
+        return face[0] == o.face[0] && face[1] == o.face[1] && face[2] == o.face[2] && node == o.node;
+    }
+};
 
 // Device-side view of one block: sizes, array pointers and per-block run parameters.
 // P, Fv and FH hold the flux type (double or float); everything else is double.
@@ -45,13 +58,16 @@ struct LaunchList {
     uint32_t nblk = 0;
     uint32_t total[4] = {};
     uint32_t grid_cap = 0;            // 0: one workgroup per tile; else persistent grid of this size
+    TileCfg tiles;
 };
 
 // kernels (block_kernels.hip)
+// ev (optional): 6 events recorded before the viscous kernel, after it, after each face sweep
+// and after the update
 void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev);
 void base(const BlockDev &B, int precision, hipStream_t st);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
-uint32_t tiles_of(const BlockDev &B, int kind);
+uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg);
 
 // device-side set-up (init_kernels.hip)
 void init_geometry(const BlockDev &B, const double L[3], double amp, const double ph[3], hipStream_t st);

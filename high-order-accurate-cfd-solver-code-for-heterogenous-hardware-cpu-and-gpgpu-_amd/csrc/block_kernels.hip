@@ -77,8 +77,8 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ pref
 // ---------------------------------------------------------------------------
 // viscous node kernel (Viscous.cl:285-631 on a periodic block: central 4th order)
 // ---------------------------------------------------------------------------
-template <typename S>
-__global__ void __launch_bounds__(NODE_TX *NODE_TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+template <typename S, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                            uint32_t nblk, uint32_t total, int mode)
 {
     for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
@@ -86,9 +86,9 @@ __global__ void __launch_bounds__(NODE_TX *NODE_TY) k_visc(const BlockDev *__res
         const BlockDev &B = blks[find_block(prefix, nblk, t)];
         if (!B.viscous) continue;
         const uint32_t lt = t - B.tile0[0];
-        const uint32_t ntx = (B.ni + NODE_TX - 1) / NODE_TX, nty = (B.nj + NODE_TY - 1) / NODE_TY;
-        const int i = (int)((lt % ntx) * NODE_TX + threadIdx.x % NODE_TX);
-        const int j = (int)(((lt / ntx) % nty) * NODE_TY + threadIdx.x / NODE_TX);
+        const uint32_t ntx = (B.ni + TX - 1) / TX, nty = (B.nj + TY - 1) / TY;
+        const int i = (int)((lt % ntx) * TX + threadIdx.x % TX);
+        const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
         if (i >= B.ni || j >= B.nj) continue;
         const uint32_t g = node_index(B, i, j, k);
@@ -144,10 +144,10 @@ __global__ void __launch_bounds__(NODE_TX *NODE_TY) k_visc(const BlockDev *__res
 // TB+5 stencil points of every line in LDS (each point computed once per tile); phase 2 gives
 // each thread one face, reading its six stencil points from LDS.
 // ---------------------------------------------------------------------------
-template <int DIR>
+template <int DIR, int TA, int TB>
 __device__ __forceinline__ int lds_at(int c, int s, int a)
 {
-    constexpr int TA = FaceTile<DIR>::TA, NP = FaceTile<DIR>::TB + 5;
+    constexpr int NP = TB + 5;
     return DIR == 0 ? (c * TA + a) * NP + s : (c * NP + s) * TA + a;
 }
 
@@ -159,11 +159,12 @@ __device__ __forceinline__ uint32_t tile_node(const BlockDev &B, int along, int 
     return node_index(B, across, fixed, along);
 }
 
-template <typename S, int DIR>
+template <typename S, int DIR, int TA, int TB>
 __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                               uint32_t nblk, uint32_t total)
 {
-    constexpr int TA = FaceTile<DIR>::TA, TB = FaceTile<DIR>::TB, NP = TB + 5;
+    static_assert(TA * TB == 256, "face tiles have 256 threads");
+    constexpr int NP = TB + 5;
     __shared__ S sq[5 * NP * TA];
     __shared__ S sf[5 * NP * TA];
     const int tid = threadIdx.x;
@@ -195,8 +196,8 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
                       S(ld(B.det, x)), f);
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
-                sq[lds_at<DIR>(c, s, a)] = q[c];
-                sf[lds_at<DIR>(c, s, a)] = f[c];
+                sq[lds_at<DIR, TA, TB>(c, s, a)] = q[c];
+                sf[lds_at<DIR, TA, TB>(c, s, a)] = f[c];
             }
         }
         __syncthreads();
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 #pragma unroll
             for (int m = 0; m < 6; ++m)
 #pragma unroll
-                for (int c = 0; c < 5; ++c) q[m][c] = sq[lds_at<DIR>(c, b + m, a)];
+                for (int c = 0; c < 5; ++c) q[m][c] = sq[lds_at<DIR, TA, TB>(c, b + m, a)];
             const uint32_t gl = tile_node<DIR>(B, b0 + b, a0 + a, fixed);
             const uint32_t gr = tile_node<DIR>(B, wrapi(b0 + b + 1, n_along), a0 + a, fixed);
             S roe[5];
@@ -223,9 +224,9 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 #pragma unroll
             for (int c = 0; c < 5; ++c)
                 st((S *)B.FH[DIR][c], gl,
-                   S(llf[c] + midpoint_correction(sf[lds_at<DIR>(c, b, a)], sf[lds_at<DIR>(c, b + 1, a)],
-                                                  sf[lds_at<DIR>(c, b + 2, a)], sf[lds_at<DIR>(c, b + 3, a)],
-                                                  sf[lds_at<DIR>(c, b + 4, a)], sf[lds_at<DIR>(c, b + 5, a)])));
+                   S(llf[c] + midpoint_correction(sf[lds_at<DIR, TA, TB>(c, b, a)], sf[lds_at<DIR, TA, TB>(c, b + 1, a)],
+                                                  sf[lds_at<DIR, TA, TB>(c, b + 2, a)], sf[lds_at<DIR, TA, TB>(c, b + 3, a)],
+                                                  sf[lds_at<DIR, TA, TB>(c, b + 4, a)], sf[lds_at<DIR, TA, TB>(c, b + 5, a)])));
         }
     }
 }
@@ -234,17 +235,17 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 // update kernel: L and the SSP-RK3 stage (Weno.cl:3060-3135), in double.
 //   mode 0..2: RK stage j; mode 3: write L to the block's Lout only
 // ---------------------------------------------------------------------------
-template <typename S>
-__global__ void __launch_bounds__(NODE_TX *NODE_TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+template <typename S, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                              uint32_t nblk, uint32_t total, int mode)
 {
     for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
         const uint32_t t = xcd_tile(wg, total);
         const BlockDev &B = blks[find_block(prefix, nblk, t)];
         const uint32_t lt = t - B.tile0[0];
-        const uint32_t ntx = (B.ni + NODE_TX - 1) / NODE_TX, nty = (B.nj + NODE_TY - 1) / NODE_TY;
-        const int i = (int)((lt % ntx) * NODE_TX + threadIdx.x % NODE_TX);
-        const int j = (int)(((lt / ntx) % nty) * NODE_TY + threadIdx.x / NODE_TX);
+        const uint32_t ntx = (B.ni + TX - 1) / TX, nty = (B.nj + TY - 1) / TY;
+        const int i = (int)((lt % ntx) * TX + threadIdx.x % TX);
+        const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
         if (i >= B.ni || j >= B.nj) continue;
         const uint32_t g = node_index(B, i, j, k);
@@ -345,20 +346,68 @@ __global__ void k_copy_prims(BlockDev B, double *__restrict__ out)
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+template <typename S, int V>
+void launch_face_i(const LaunchList &L, dim3 g, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_face<S, 0, FACE_I_TILES[V][0], FACE_I_TILES[V][1]>), g, dim3(256), 0, st, L.blks, L.prefix[1],
+                       L.nblk, L.total[1]);
+}
+template <typename S, int DIR, int V>
+void launch_face_jk(const LaunchList &L, dim3 g, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_face<S, DIR, FACE_JK_TILES[V][0], FACE_JK_TILES[V][1]>), g, dim3(256), 0, st, L.blks,
+                       L.prefix[1 + DIR], L.nblk, L.total[1 + DIR]);
+}
+template <typename S, int V>
+void launch_node(const LaunchList &L, dim3 g, hipStream_t st, bool visc_kernel, int mode)
+{
+    constexpr int TX = NODE_TILES[V][0], TY = NODE_TILES[V][1];
+    if (visc_kernel) hipLaunchKernelGGL((k_visc<S, TX, TY>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    else hipLaunchKernelGGL((k_update<S, TX, TY>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+}
+
+template <typename S>
+void node_kernel(const LaunchList &L, dim3 g, hipStream_t st, bool visc_kernel, int mode)
+{
+    switch (L.tiles.node) {
+    case 1: launch_node<S, 1>(L, g, st, visc_kernel, mode); break;
+    case 2: launch_node<S, 2>(L, g, st, visc_kernel, mode); break;
+    case 3: launch_node<S, 3>(L, g, st, visc_kernel, mode); break;
+    default: launch_node<S, 0>(L, g, st, visc_kernel, mode);
+    }
+}
+
+template <typename S, int DIR>
+void face_jk(const LaunchList &L, dim3 g, hipStream_t st)
+{
+    switch (L.tiles.face[DIR]) {
+    case 1: launch_face_jk<S, DIR, 1>(L, g, st); break;
+    case 2: launch_face_jk<S, DIR, 2>(L, g, st); break;
+    case 3: launch_face_jk<S, DIR, 3>(L, g, st); break;
+    default: launch_face_jk<S, DIR, 0>(L, g, st);
+    }
+}
+
 template <typename S>
 void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev)
 {
     auto rec = [&](int s) { if (ev) (void)hipEventRecord(ev[s], st); };
     auto grid = [&](uint32_t total) { return dim3(L.grid_cap ? std::min(total, L.grid_cap) : total); };
     rec(0);
-    if (any_visc) hipLaunchKernelGGL((k_visc<S>), grid(L.total[0]), dim3(NODE_TX * NODE_TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    if (any_visc) node_kernel<S>(L, grid(L.total[0]), st, true, mode);
     rec(1);
-    hipLaunchKernelGGL((k_face<S, 0>), grid(L.total[1]), dim3(256), 0, st, L.blks, L.prefix[1], L.nblk, L.total[1]);
-    hipLaunchKernelGGL((k_face<S, 1>), grid(L.total[2]), dim3(256), 0, st, L.blks, L.prefix[2], L.nblk, L.total[2]);
-    hipLaunchKernelGGL((k_face<S, 2>), grid(L.total[3]), dim3(256), 0, st, L.blks, L.prefix[3], L.nblk, L.total[3]);
+    switch (L.tiles.face[0]) {
+    case 1: launch_face_i<S, 1>(L, grid(L.total[1]), st); break;
+    case 2: launch_face_i<S, 2>(L, grid(L.total[1]), st); break;
+    default: launch_face_i<S, 0>(L, grid(L.total[1]), st);
+    }
     rec(2);
-    hipLaunchKernelGGL((k_update<S>), grid(L.total[0]), dim3(NODE_TX * NODE_TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    face_jk<S, 1>(L, grid(L.total[2]), st);
     rec(3);
+    face_jk<S, 2>(L, grid(L.total[3]), st);
+    rec(4);
+    node_kernel<S>(L, grid(L.total[0]), st, false, mode);
+    rec(5);
 }
 
 void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev)
@@ -381,15 +430,18 @@ void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
     else hipLaunchKernelGGL(k_copy_prims, dim3(nb), dim3(256), 0, st, B, out);
 }
 
-uint32_t tiles_of(const BlockDev &B, int kind)
+uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg)
 {
-    if (kind == 0) return (uint32_t)(((B.ni + NODE_TX - 1) / NODE_TX) * ((B.nj + NODE_TY - 1) / NODE_TY) * B.nk);
-    auto f = [&](int TA, int TB, int along, int across, int fix) {
-        return (uint32_t)(((across + TA - 1) / TA) * ((along + TB - 1) / TB) * fix);
+    if (kind == 0) {
+        const int TX = NODE_TILES[cfg.node][0], TY = NODE_TILES[cfg.node][1];
+        return (uint32_t)(((B.ni + TX - 1) / TX) * ((B.nj + TY - 1) / TY) * B.nk);
+    }
+    auto f = [&](const int (&t)[2], int along, int across, int fix) {
+        return (uint32_t)(((across + t[0] - 1) / t[0]) * ((along + t[1] - 1) / t[1]) * fix);
     };
-    if (kind == 1) return f(FaceTile<0>::TA, FaceTile<0>::TB, B.ni, B.nj, B.nk);
-    if (kind == 2) return f(FaceTile<1>::TA, FaceTile<1>::TB, B.nj, B.ni, B.nk);
-    return f(FaceTile<2>::TA, FaceTile<2>::TB, B.nk, B.ni, B.nj);
+    if (kind == 1) return f(FACE_I_TILES[cfg.face[0]], B.ni, B.nj, B.nk);
+    if (kind == 2) return f(FACE_JK_TILES[cfg.face[1]], B.nj, B.ni, B.nk);
+    return f(FACE_JK_TILES[cfg.face[2]], B.nk, B.ni, B.nj);
 }
 
 }  // namespace blk

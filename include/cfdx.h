@@ -199,6 +199,23 @@ int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *L_host);
 int cfdx_block_set_timing(cfdx_block *b, int on);
 int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face, double *ms_update, int *launches);
 
+/* Per-kernel device time of the timed steps: ms[0] viscous, ms[1..3] face sweeps i, j, k,
+ * ms[4] update. */
+int cfdx_block_timing_split(cfdx_block *b, double ms[5]);
+
+/* Tile shapes. Every kernel class has a few compiled tile variants; kind 0 = face sweep along i
+ * (shape = lines across j, faces along i), kind 1 = face sweeps along j and k (lines across i,
+ * faces along j or k), kind 2 = node kernels (nodes along i, along j). cfdx_tile_variants gives
+ * the count and, if shapes is not NULL, 2 * count shape numbers. A block or batch launches the
+ * variants its cfdx_tiles names (all 0 by default); results do not depend on them. */
+typedef struct {
+    int32_t face[3];   /* variant of the i, j, k face sweeps */
+    int32_t node;      /* variant of the node kernels */
+} cfdx_tiles;
+int cfdx_tile_variants(int kind, int32_t *count, int32_t *shapes);
+int cfdx_block_set_tiles(cfdx_block *b, const cfdx_tiles *t);
+int cfdx_block_get_tiles(cfdx_block *b, cfdx_tiles *t);
+
 /* Batches: blocks of different sizes and run parameters stepped together. Each kernel class is
  * one launch over the tiles of every block; a step can be replayed from a captured hipGraph.
  * A batch references its blocks (it does not own them); all share its precision and device. */
@@ -214,6 +231,8 @@ int cfdx_batch_set_grid(cfdx_batch *q, uint32_t grid_cap);
 /* nsteps RK3 steps of every block; use_graph = 1 captures one step in a hipGraph on first use
  * (re-captured after the batch changes) and replays it. Asynchronous on the context stream. */
 int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph);
+/* Tile variants of the batch's launches (the blocks' own settings apply to single-block steps). */
+int cfdx_batch_set_tiles(cfdx_batch *q, const cfdx_tiles *t);
 
 /* ------------------------------------------------------------------------ */
 /* 3. Host geometry (CPU)                                                    */

@@ -1,0 +1,119 @@
+"""Tile variants and the autotuner (GPU).
+
+Results may not depend on the tile shapes: every compiled variant of every kernel class, on a
+block whose sizes are not multiples of any tile (partial tiles in all directions), and in a
+batch, gives bitwise the state of the default tiles. The tuner picks valid variants and its
+cache reproduces the choice.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cfdx
+from cfdx import tune
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if cfdx.device_count() < 1:
+        pytest.skip("no HIP device")
+    return cfdx.Ctx(0)
+
+
+def ragged(ctx, precision, ni=19, nj=13, nk=11):
+    b = cfdx.Block(ni, nj, nk, precision=precision, ctx=ctx)
+    b.init_case(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * 2 * math.pi / ni, phases=(0.4, 1.0, 2.2),
+                init="tgv", mach=0.2)
+    return b
+
+
+def all_configs():
+    counts = [len(cfdx.tile_variants(0)), len(cfdx.tile_variants(1)), len(cfdx.tile_variants(1)),
+              len(cfdx.tile_variants(2))]
+    cfgs = []
+    for cls, n in enumerate(counts):
+        for v in range(1, n):
+            c = [0, 0, 0, 0]
+            c[cls] = v
+            cfgs.append(tuple(c))
+    cfgs.append((counts[0] - 1, counts[1] - 1, counts[2] - 1, counts[3] - 1))
+    return cfgs
+
+
+def test_variant_table():
+    fi, fjk, node = cfdx.tile_variants(0), cfdx.tile_variants(1), cfdx.tile_variants(2)
+    assert fi[0] == (4, 64) and fjk[0] == (16, 16) and node[0] == (64, 4)
+    assert all(a * b == 256 for a, b in fi + fjk + node)
+
+
+@pytest.mark.parametrize("precision", [cfdx.FP64, cfdx.MIXED], ids=["fp64", "mixed"])
+def test_results_independent_of_tiles(ctx, precision):
+    prm = cfdx.default_params(mach=0.2, reyl=900.0)
+    ref = ragged(ctx, precision)
+    dt = ref.cfl_dt(0.5)
+    ref.step(2, 0.2, dt, prm)
+    want = ref.get_flow()
+    for cfg in all_configs():
+        b = ragged(ctx, precision)
+        b.set_tiles(cfg)
+        assert b.get_tiles() == cfg
+        b.step(2, 0.2, dt, prm)
+        got = b.get_flow()
+        for f in FIELDS:
+            assert np.array_equal(got[f], want[f]), (cfg, f)
+        b.close()
+
+
+def test_batch_tiles_and_grid(ctx):
+    prm = cfdx.default_params(mach=0.2, reyl=900.0)
+    shapes = [(19, 13, 11), (8, 9, 10), (17, 5, 7)]
+    alone = []
+    for s in shapes:
+        b = ragged(ctx, cfdx.MIXED, *s)
+        dt = b.cfl_dt(0.5)
+        b.step(3, 0.2, dt, prm)
+        alone.append(b.get_flow())
+    for cfg, cap in [((1, 2, 3, 1), 0), ((2, 1, 2, 3), 7), ((0, 3, 1, 2), 64)]:
+        q = cfdx.Batch(cfdx.MIXED, ctx)
+        blocks = []
+        for s in shapes:
+            b = ragged(ctx, cfdx.MIXED, *s)
+            b.configure(0.2, b.cfl_dt(0.5), prm)
+            q.add(b)
+            blocks.append(b)
+        q.set_tiles(cfg)
+        q.set_grid(cap)
+        q.step(3, True)
+        for b, want in zip(blocks, alone):
+            got = b.get_flow()
+            for f in FIELDS:
+                assert np.array_equal(got[f], want[f]), (cfg, cap, f)
+        q.close()
+
+
+def test_tuner_and_cache(ctx, tmp_path):
+    cache = tune.TileCache(str(tmp_path / "tiles.json"))
+    prm = cfdx.default_params(mach=0.2, reyl=900.0)
+    b = ragged(ctx, cfdx.MIXED, 32, 24, 16)
+    dt = b.cfl_dt(0.5)
+    res = tune.tune_block(b, 0.2, dt, prm, steps=1, cache=cache)
+    assert not res["cached"] and b.get_tiles() == tuple(res["tiles"])
+    assert set(res["times_ms"]) == {"face_i", "face_j", "face_k", "node"}
+    assert all(t > 0 for ts in res["times_ms"].values() for t in ts)
+    again = tune.TileCache(str(tmp_path / "tiles.json"))
+    b2 = ragged(ctx, cfdx.MIXED, 32, 24, 16)
+    res2 = tune.tune_block(b2, 0.2, dt, prm, cache=again)
+    assert res2["cached"] and b2.get_tiles() == tuple(res["tiles"])
+    q = cfdx.Batch(cfdx.MIXED, ctx)
+    b3 = ragged(ctx, cfdx.MIXED, 24, 16, 8)
+    b3.configure(0.2, b3.cfl_dt(0.5), prm)
+    q.add(b)
+    q.add(b3)
+    rq = tune.tune_batch(q, ctx, key="test-batch", steps=1, cache=cache)
+    assert rq["grid_cap"] in tune.GRID_CAPS and len(rq["tiles"]) == 4
+    q.close()
