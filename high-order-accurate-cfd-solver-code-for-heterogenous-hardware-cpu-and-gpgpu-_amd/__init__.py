@@ -127,6 +127,12 @@ EXPORTS = {
     "cfdx_block_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_block_get_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_batch_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
+    "cfdx_block_set_monitor": (C.c_int, [_vp, C.c_int]),
+    "cfdx_block_residual": (C.c_int, [_vp, C.POINTER(_d)]),
+    "cfdx_write_restart": (C.c_int, [C.c_char_p, _i32, _i64, _vp, _i64, _i32]),
+    "cfdx_restart_latest": (C.c_int, [C.c_char_p, _i64, C.POINTER(_i32), C.POINTER(_i64)]),
+    "cfdx_read_restart": (C.c_int, [C.c_char_p, _i32, _i64, _vp, C.POINTER(_i64)]),
+    "cfdx_write_tecplot": (C.c_int, [C.c_char_p, _i32, _i32, _i32, _vp, _vp, _vp, _d]),
     "cfdx_geom_periodic_table": (C.c_int, [_i32, _i32, _i32, _vp, _vp]),
     "cfdx_geom_coords_box": (C.c_int, [_i32, _i32, _i32, _vp, _vp]),
     "cfdx_geom_coords_sinusoid": (C.c_int, [_i32, _i32, _i32, _vp, _d, _vp, _vp]),
@@ -478,6 +484,15 @@ class Block:
     def set_timing(self, on: bool) -> None:
         _check(_lib.cfdx_block_set_timing(self.h, int(on)))
 
+    def set_monitor(self, on: bool = True) -> None:
+        _check(_lib.cfdx_block_set_monitor(self.h, int(on)))
+
+    def residual(self) -> tuple[float, float, float]:
+        """max |du|, |dv|, |de| of the last step (monitor on)."""
+        out = (_d * 3)()
+        _check(_lib.cfdx_block_residual(self.h, out))
+        return out[0], out[1], out[2]
+
     def timing_split(self) -> list[float]:
         """ms of the timed steps: viscous, face i, face j, face k, update."""
         ms = (_d * 5)()
@@ -539,6 +554,37 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# on-disk formats
+# ---------------------------------------------------------------------------
+def write_restart(directory: str, slot: int, flow: np.ndarray, iteration: int, digits: int = 17) -> None:
+    flow = np.ascontiguousarray(flow)
+    assert flow.dtype == FLOW_DT
+    _check(_lib.cfdx_write_restart(directory.encode(), slot, flow.shape[0], _ptr(flow), iteration, digits))
+
+
+def restart_latest(directory: str, n: int) -> tuple[int, int]:
+    """(slot, iteration) of the newer complete restart file; (0, -1) when none."""
+    slot, it = _i32(0), _i64(0)
+    _check(_lib.cfdx_restart_latest(directory.encode(), n, C.byref(slot), C.byref(it)))
+    return slot.value, it.value
+
+
+def read_restart(directory: str, slot: int, n: int) -> tuple[np.ndarray, int]:
+    flow = np.zeros(n, dtype=FLOW_DT)
+    it = _i64(0)
+    _check(_lib.cfdx_read_restart(directory.encode(), slot, n, _ptr(flow), C.byref(it)))
+    return flow, it.value
+
+
+def write_tecplot(path: str, ni: int, nj: int, nk: int, xyz: np.ndarray, flow: np.ndarray, loc=None,
+                  solution_time: float = 0.0) -> None:
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    flow = np.ascontiguousarray(flow)
+    locp = None if loc is None else np.ascontiguousarray(loc, dtype=np.int32)
+    _check(_lib.cfdx_write_tecplot(path.encode(), ni, nj, nk, _ptr(xyz), _ptr(flow), _ptr(locp), solution_time))
 
 
 def exported_symbols() -> list[str]:

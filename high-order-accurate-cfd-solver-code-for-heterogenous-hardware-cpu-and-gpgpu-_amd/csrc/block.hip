@@ -30,6 +30,9 @@ struct cfdx_block {
     double t_visc = 0, t_face[3] = {0, 0, 0}, t_update = 0;
     int launches = 0;
     cfdx::blk::TileCfg tiles;
+    double *mon = nullptr;        // residual monitor partials (3 doubles per wave) + 3 reduced values
+    uint32_t mon_cap = 0;         // partial triples allocated
+    uint32_t mon_count = 0;       // partial triples written by the last step
 };
 
 struct cfdx_batch {
@@ -113,9 +116,16 @@ LaunchList single_list(cfdx_block *b)
     return L;
 }
 
+uint32_t mon_triples(const BlockDev &B, const cfdx::blk::TileCfg &t)
+{
+    const int tx = cfdx::blk::NODE_TILES[t.node][0], ty = cfdx::blk::NODE_TILES[t.node][1];
+    return cfdx::blk::tiles_of(B, 0, t) * (uint32_t)(tx * ty / 64);
+}
+
 int run_stage(cfdx_block *b, int mode)
 {
     const LaunchList L = single_list(b);
+    if (mode == 2) b->mon_count = mon_triples(b->B, b->tiles);
     cfdx::blk::stage(L, b->desc.precision, mode, b->ctx->stream, b->B.viscous != 0, b->timing ? b->ev : nullptr);
     b->launches += (b->B.viscous ? 5 : 4);
     CFDX_TRY(cfdx::check_launch("block stage"));
@@ -264,6 +274,7 @@ extern "C" int cfdx_block_destroy(cfdx_block *b)
     if (b->mem) (void)hipFree(b->mem);
     if (b->dB) (void)hipFree(b->dB);
     if (b->dprefix) (void)hipFree(b->dprefix);
+    if (b->mon) (void)hipFree(b->mon);
     cfdx_ctx *ctx = b->ctx;
     delete b;
     return cfdx::ctx_release(ctx);
@@ -454,6 +465,41 @@ extern "C" int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face
     return CFDX_OK;
 }
 
+extern "C" int cfdx_block_set_monitor(cfdx_block *b, int on)
+{
+    if (!b) return cfdx::fail(CFDX_E_INVALID, "set_monitor: null block");
+    CFDX_TRY(cfdx::enter(b->ctx));
+    if (on && !b->mon) {
+        uint32_t cap = 0;
+        for (int v = 0; v < cfdx::blk::N_NODE; ++v) {
+            cfdx::blk::TileCfg t;
+            t.node = v;
+            cap = std::max(cap, mon_triples(b->B, t));
+        }
+        CFDX_HIP(hipMalloc((void **)&b->mon, (size_t)(cap + 1) * 3 * sizeof(double)), "set_monitor hipMalloc");
+        CFDX_HIP(hipMemsetAsync(b->mon, 0, (size_t)(cap + 1) * 3 * sizeof(double), b->ctx->stream), "set_monitor clear");
+        b->mon_cap = cap;
+        b->mon_count = 0;
+    }
+    double *want = on ? b->mon : nullptr;
+    if (b->B.mon != want) { b->B.mon = want; ++b->version; }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_residual(cfdx_block *b, double out[3])
+{
+    if (!b || !out) return cfdx::fail(CFDX_E_INVALID, "residual: null argument");
+    if (!b->B.mon) return cfdx::fail(CFDX_E_INVALID, "residual: the monitor is off (cfdx_block_set_monitor)");
+    if (b->mon_count == 0) return cfdx::fail(CFDX_E_INVALID, "residual: no step with the monitor on yet");
+    CFDX_TRY(cfdx::enter(b->ctx));
+    double *dout = b->mon + 3 * (size_t)b->mon_cap;
+    cfdx::blk::monitor_reduce(b->mon, b->mon_count, dout, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("monitor_reduce"));
+    CFDX_HIP(hipMemcpyAsync(out, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, b->ctx->stream), "residual copy");
+    CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "residual sync");
+    return CFDX_OK;
+}
+
 extern "C" int cfdx_block_timing_split(cfdx_block *b, double ms[5])
 {
     if (!b || !ms) return cfdx::fail(CFDX_E_INVALID, "timing_split: null argument");
@@ -585,6 +631,7 @@ extern "C" int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph)
     if (nsteps < 0) return cfdx::fail(CFDX_E_INVALID, "batch_step: nsteps < 0");
     if (q->blocks.empty() || nsteps == 0) return CFDX_OK;
     CFDX_TRY(upload_batch(q));
+    for (cfdx_block *b : q->blocks) b->mon_count = mon_triples(b->B, q->tiles);
     hipStream_t st = q->ctx->stream;
     if (!use_graph) {
         for (int s = 0; s < nsteps; ++s)

@@ -46,6 +46,7 @@ struct BlockDev {
     double *mu;
     void *FH[3][5];             // face fluxes F-hat at the +1/2 faces
     double *Lout;               // RHS output (mode 3)
+    double *mon;                // residual monitor: per-wave max |du|, |dv|, |de| of the last step (null: off)
     double del_t, mach, eps;
     int viscous;
     cfdx::dev::ViscParams vp;
@@ -68,6 +69,8 @@ void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool an
 void base(const BlockDev &B, int precision, hipStream_t st);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
 uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg);
+// out[0..2] (device) = maxima of count per-wave monitor triples
+void monitor_reduce(const double *mon, uint32_t count, double *out, hipStream_t st);
 
 // device-side set-up (init_kernels.hip)
 void init_geometry(const BlockDev &B, const double L[3], double amp, const double ph[3], hipStream_t st);

@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 // ---------------------------------------------------------------------------
 template <typename S, int TX, int TY>
 __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
-                                                             uint32_t nblk, uint32_t total, int mode)
+                                                   uint32_t nblk, uint32_t total, int mode)
 {
     for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
         const uint32_t t = xcd_tile(wg, total);
@@ -247,64 +247,106 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
         const int i = (int)((lt % ntx) * TX + threadIdx.x % TX);
         const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
-        if (i >= B.ni || j >= B.nj) continue;
-        const uint32_t g = node_index(B, i, j, k);
-        double L[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const uint32_t m1 = shifted(B, i, j, k, d, -1);
-#pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                const S *fh = (const S *)B.FH[d][c];
-                L[c] += (double)ld(fh, g) - (double)ld(fh, m1);
-            }
-        }
-        if (B.viscous) {
+        // residual monitor of the step (Solver.cpp:546-573): |du|, |dv|, |de| against the RK base
+        double mdu = 0.0, mdv = 0.0, mde = 0.0;
+        if (i < B.ni && j < B.nj) {
+            const uint32_t g = node_index(B, i, j, k);
+            double L[5] = {0, 0, 0, 0, 0};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
-                const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+                const uint32_t m1 = shifted(B, i, j, k, d, -1);
 #pragma unroll
-                for (int c = 1; c < 5; ++c) {
-                    const S *f = (const S *)B.Fv[d][c - 1];
-                    L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
+                for (int c = 0; c < 5; ++c) {
+                    const S *fh = (const S *)B.FH[d][c];
+                    L[c] += (double)ld(fh, g) - (double)ld(fh, m1);
+                }
+            }
+            if (B.viscous) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
+                    const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+#pragma unroll
+                    for (int c = 1; c < 5; ++c) {
+                        const S *f = (const S *)B.Fv[d][c - 1];
+                        L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) L[c] = -L[c];
+            if (mode == 3) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) B.Lout[(size_t)c * B.n + g] = L[c];
+            } else {
+                const double del_t = B.del_t;
+                double un[5], u0[5];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) u0[c] = ld(B.U0[c], g);
+                if (mode == 0) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) { un[c] = u0[c] + del_t * L[c]; st(B.Us[c], g, un[c]); }
+                } else if (mode == 1) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) { un[c] = 0.75 * u0[c] + 0.25 * ld(B.Us[c], g) + 0.25 * del_t * L[c]; st(B.Us[c], g, un[c]); }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) un[c] = (u0[c] * (1.0 / 3.0)) + ((2.0 / 3.0) * ld(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
+                }
+                const double det = ld(B.det, g);
+                const double idet = 1.0 / det;
+                const double rho = idet * un[0], ir = 1.0 / rho;
+                const double u = (idet * un[1]) * ir, v = (idet * un[2]) * ir, w = (idet * un[3]) * ir;
+                const double e = (idet * un[4]) * ir - 0.5 * (u * u + v * v + w * w);
+                const double p = e * rho * 0.4;
+                st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
+                if (mode == 2) {
+                    // next step's RK base: U_Q recomputes det*Q from the new primitives (Solver.cpp:377-388)
+                    double q[5];
+                    prim_to_q(rho, u, v, w, p, q);
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
+                    if (B.mon) {
+                        const double r0 = idet * u0[0], ir0 = 1.0 / r0;
+                        const double uo = (idet * u0[1]) * ir0, vo = (idet * u0[2]) * ir0, wo = (idet * u0[3]) * ir0;
+                        const double eo = (idet * u0[4]) * ir0 - 0.5 * (uo * uo + vo * vo + wo * wo);
+                        mdu = fabs(u - uo); mdv = fabs(v - vo); mde = fabs(e - eo);
+                    }
                 }
             }
         }
+        if (mode == 2 && B.mon) {
+            // one maximum per wave (every lane takes part), written by lane 0
 #pragma unroll
-        for (int c = 0; c < 5; ++c) L[c] = -L[c];
-        if (mode == 3) {
-#pragma unroll
-            for (int c = 0; c < 5; ++c) B.Lout[(size_t)c * B.n + g] = L[c];
-            continue;
-        }
-        const double del_t = B.del_t;
-        double un[5];
-        if (mode == 0) {
-#pragma unroll
-            for (int c = 0; c < 5; ++c) { un[c] = ld(B.U0[c], g) + del_t * L[c]; st(B.Us[c], g, un[c]); }
-        } else if (mode == 1) {
-#pragma unroll
-            for (int c = 0; c < 5; ++c) { un[c] = 0.75 * ld(B.U0[c], g) + 0.25 * ld(B.Us[c], g) + 0.25 * del_t * L[c]; st(B.Us[c], g, un[c]); }
-        } else {
-#pragma unroll
-            for (int c = 0; c < 5; ++c) un[c] = (ld(B.U0[c], g) * (1.0 / 3.0)) + ((2.0 / 3.0) * ld(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
-        }
-        const double det = ld(B.det, g);
-        const double idet = 1.0 / det;
-        const double rho = idet * un[0], ir = 1.0 / rho;
-        const double u = (idet * un[1]) * ir, v = (idet * un[2]) * ir, w = (idet * un[3]) * ir;
-        const double e = (idet * un[4]) * ir - 0.5 * (u * u + v * v + w * w);
-        const double p = e * rho * 0.4;
-        st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
-        if (mode == 2) {
-            // next step's RK base: U_Q recomputes det*Q from the new primitives (Solver.cpp:377-388)
-            double q[5];
-            prim_to_q(rho, u, v, w, p, q);
-#pragma unroll
-            for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
+            for (int o = 32; o > 0; o >>= 1) {
+                mdu = fmax(mdu, __shfl_xor(mdu, o));
+                mdv = fmax(mdv, __shfl_xor(mdv, o));
+                mde = fmax(mde, __shfl_xor(mde, o));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                double *m = B.mon + 3 * ((size_t)lt * (TX * TY / 64) + threadIdx.x / 64);
+                m[0] = mdu; m[1] = mdv; m[2] = mde;
+            }
         }
     }
+}
+
+// maxima over the per-wave monitor values: out[0..2] = max |du|, |dv|, |de|
+__global__ void __launch_bounds__(256) k_mon_reduce(const double *__restrict__ mon, uint32_t count, double *__restrict__ out)
+{
+    __shared__ double sh[3][256];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t q = threadIdx.x; q < count; q += 256) {
+        a = fmax(a, mon[3 * q]); b = fmax(b, mon[3 * q + 1]); c = fmax(c, mon[3 * q + 2]);
+    }
+    sh[0][threadIdx.x] = a; sh[1][threadIdx.x] = b; sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int r = 0; r < 3; ++r) sh[r][threadIdx.x] = fmax(sh[r][threadIdx.x], sh[r][threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // set_flow: double primitives staged in Us -> P (type S) and U0 = det * Q(P) in double
@@ -428,6 +470,11 @@ void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
     const unsigned nb = (unsigned)((B.n + 255) / 256);
     if (precision == CFDX_MIXED) hipLaunchKernelGGL(k_prims, dim3(nb), dim3(256), 0, st, B, out);
     else hipLaunchKernelGGL(k_copy_prims, dim3(nb), dim3(256), 0, st, B, out);
+}
+
+void monitor_reduce(const double *mon, uint32_t count, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mon_reduce, dim3(1), dim3(256), 0, st, mon, count, out);
 }
 
 uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg)

@@ -68,9 +68,12 @@ template <> __device__ __forceinline__ double prsq_<double>(double x)
 template <> __device__ __forceinline__ float prsq_<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Conserved state Q = (rho, rho u, rho v, rho w, p/0.4 + rho |u|^2 / 2) of primitives (U_Q.cl:118-122).
+// No fused multiply-adds, so every kernel that builds the RK base det * Q from the same primitives
+// (the last RK stage, set_flow) gets the same bits: a state saved and restored continues exactly.
 template <typename T>
 __device__ __forceinline__ void prim_to_q(T rho, T u, T v, T w, T p, T (&q)[5])
 {
+#pragma clang fp contract(off)
     const T ke = T(0.5) * rho * (u * u + v * v + w * w);
     q[0] = rho;
     q[1] = rho * u;

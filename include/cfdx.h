@@ -199,6 +199,14 @@ int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *L_host);
 int cfdx_block_set_timing(cfdx_block *b, int on);
 int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face, double *ms_update, int *launches);
 
+/* Residual monitor (Solver.cpp:546-573): with the monitor on, every step also records the
+ * maxima over the block of |u_new - u_old|, |v_new - v_old| and |e_new - e_old| (e = internal
+ * energy p / (0.4 rho)) between the start and the end of the step; cfdx_block_residual returns
+ * those of the last step (out[0..2], synchronous). set_monitor(on) allocates the partial
+ * maxima on first use. */
+int cfdx_block_set_monitor(cfdx_block *b, int on);
+int cfdx_block_residual(cfdx_block *b, double out[3]);
+
 /* Per-kernel device time of the timed steps: ms[0] viscous, ms[1..3] face sweeps i, j, k,
  * ms[4] update. */
 int cfdx_block_timing_split(cfdx_block *b, double ms[5]);
@@ -233,6 +241,24 @@ int cfdx_batch_set_grid(cfdx_batch *q, uint32_t grid_cap);
 int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph);
 /* Tile variants of the batch's launches (the blocks' own settings apply to single-block steps). */
 int cfdx_batch_set_tiles(cfdx_batch *q, const cfdx_tiles *t);
+
+/* ------------------------------------------------------------------------ */
+/* On-disk formats (host)                                                    */
+/* ------------------------------------------------------------------------ */
+/* Restart files <dir>/restart_file_<slot>.neu, slot 1 or 2 written in turn, one line per node
+ * "u v w rho p t e mu iter" tab-separated with `digits` significant digits (the reference writes 6,
+ * Tecplot_output_files.cpp:115-127; 17 round-trips doubles). */
+int cfdx_write_restart(const char *dir, int32_t slot, int64_t n, const cfdx_flow *flow, int64_t iter, int32_t digits);
+/* The slot (1 or 2) of the newer complete restart file of n nodes and its iteration; slot 0 and
+ * iter -1 when neither is complete (presolver.cpp:86-181). */
+int cfdx_restart_latest(const char *dir, int64_t n, int32_t *slot, int64_t *iter);
+/* Reads n nodes of a restart file (fails unless complete); flow.a is left 0. */
+int cfdx_read_restart(const char *dir, int32_t slot, int64_t n, cfdx_flow *flow, int64_t *iter);
+/* Tecplot ASCII point file of a structured block with the reference's variable list
+ * (X Y Z U V W RHO P T e mu loc Proc DUCROS) and FEBRICK cells (Tecplot_output_files.cpp:92-113);
+ * loc may be NULL (all 0). */
+int cfdx_write_tecplot(const char *path, int32_t ni, int32_t nj, int32_t nk, const double *xyz, const cfdx_flow *flow,
+                       const int32_t *loc, double solution_time);
 
 /* ------------------------------------------------------------------------ */
 /* 3. Host geometry (CPU)                                                    */

@@ -1,0 +1,69 @@
+"""Residual monitor and the iteration driver (GPU)."""
+import math
+
+import numpy as np
+import pytest
+
+import cfdx
+from cfdx import driver
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if cfdx.device_count() < 1:
+        pytest.skip("no HIP device")
+    return cfdx.Ctx(0)
+
+
+def tgv(ctx, precision=cfdx.FP64, n=(20, 16, 12)):
+    b = cfdx.Block(*n, precision=precision, ctx=ctx)
+    b.init_case(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * 2 * math.pi / n[0], phases=(0.2, 0.9, 1.7),
+                init="tgv", mach=0.3)
+    return b
+
+
+@pytest.mark.parametrize("precision", [cfdx.FP64, cfdx.MIXED], ids=["fp64", "mixed"])
+def test_monitor_matches_host_difference(ctx, precision):
+    prm = cfdx.default_params(mach=0.3, reyl=800.0)
+    b = tgv(ctx, precision)
+    dt = b.cfl_dt(0.5)
+    b.set_monitor(True)
+    b.set_tiles((0, 1, 2, 3))   # partial tiles along i with the (256 x 1) node tile
+    for _ in range(3):
+        before = b.get_flow()
+        b.step(1, 0.3, dt, prm)
+        after = b.get_flow()
+        du, dv, de = b.residual()
+        e0 = before["p"] / (0.4 * before["rho"])
+        e1 = after["p"] / (0.4 * after["rho"])
+        want = (np.abs(after["u"] - before["u"]).max(), np.abs(after["v"] - before["v"]).max(), np.abs(e1 - e0).max())
+        tol = 1e-9 if precision == cfdx.FP64 else 1e-4
+        for got, w in zip((du, dv, de), want):
+            assert abs(got - w) <= tol * w, (got, w)
+
+
+def test_driver_restart_resume(ctx, tmp_path):
+    prm = cfdx.default_params(mach=0.3, reyl=800.0)
+    ref = tgv(ctx)
+    dt = ref.cfl_dt(0.5)
+    lines = []
+    r = driver.Run(ref, 0.3, dt, prm, outdir=str(tmp_path / "a"), restart_every=2, tecplot_every=3,
+                   xyz=np.zeros((ref.n, 3)), log=lines.append)
+    r.run(5)
+    assert len(r.history) == 5 and lines[0].startswith("iteration = 1 max_div_u = ")
+    want = ref.get_flow()
+    # restart files in turn: slot 1 at step 2, slot 2 at step 4
+    assert cfdx.restart_latest(str(tmp_path / "a"), ref.n) == (2, 4)
+    # resume from step 4 in a new block and finish step 5: the same state bit for bit
+    b = tgv(ctx)
+    r2 = driver.Run(b, 0.3, dt, prm, outdir=str(tmp_path / "a"), restart_every=2, log=lines.append)
+    assert r2.resume() == 4
+    r2.run(5)
+    got = b.get_flow()
+    for f in FIELDS:
+        assert np.array_equal(got[f], want[f]), f
+    assert (tmp_path / "a" / "nodefile_3.dat").exists()
