@@ -42,9 +42,19 @@ class Params(C.Structure):
                 ("precision", C.c_int32), ("viscous", C.c_int32)]
 
 
+BC_WALL, BC_FARFIELD, BC_OUTFLOW, BC_EXTRAP, BC_WALL_CUT = 1, 2, 3, 4, 5
+
+
 class BlockDesc(C.Structure):
     _fields_ = [("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32), ("periodic", C.c_int32),
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("bc", C.c_int32 * 6), ("cut_i0", C.c_int32), ("cut_i1", C.c_int32),
+                ("aoa", C.c_double)]
+
+
+class CGrid(C.Structure):
+    _fields_ = [("m", C.c_double), ("p", C.c_double), ("t", C.c_double), ("wake", C.c_double),
+                ("radius", C.c_double), ("wall", C.c_double), ("span", C.c_double), ("n_wake", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 INIT_KINDS = {"freestream": 0, "vortex": 1, "tgv": 2}
@@ -106,6 +116,12 @@ EXPORTS = {
     "cfdx_block_create": (C.c_int, [_vp, C.POINTER(BlockDesc), C.POINTER(_vp)]),
     "cfdx_block_destroy": (C.c_int, [_vp]),
     "cfdx_block_set_geometry": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "cfdx_block_set_geometry_soa": (C.c_int, [_vp, _vp]),
+    "cfdx_block_padded": (C.c_int, [C.POINTER(BlockDesc), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "cfdx_geom_cgrid_desc": (C.c_int, [_i32, _i32, _i32, _i32, C.POINTER(CGrid), _d, C.POINTER(BlockDesc)]),
+    "cfdx_geom_cgrid_coords": (C.c_int, [C.POINTER(BlockDesc), C.POINTER(CGrid), _vp]),
+    "cfdx_geom_box_coords": (C.c_int, [C.POINTER(BlockDesc), _vp, _d, _vp, _vp]),
+    "cfdx_geom_block_metrics": (C.c_int, [C.POINTER(BlockDesc), _vp, _vp, _vp]),
     "cfdx_block_set_flow": (C.c_int, [_vp, _vp]),
     "cfdx_block_get_flow": (C.c_int, [_vp, _vp]),
     "cfdx_block_init_case": (C.c_int, [_vp, C.POINTER(CaseDesc)]),
@@ -265,6 +281,70 @@ def periodic_case(ni: int, nj: int, nk: int, grid: str = "box", length=(10.0, 10
                 FLOW=flow)
 
 
+# ---------------------------------------------------------------------------
+# bounded blocks: padded layout, C-grid and box geometry
+# ---------------------------------------------------------------------------
+def padded(desc: BlockDesc) -> tuple[tuple, tuple]:
+    """(stored sizes, owned offsets) of a descriptor: 3 ghost layers per side of a bounded direction."""
+    dims, off = (C.c_int32 * 3)(), (C.c_int32 * 3)()
+    _check(_lib.cfdx_block_padded(C.byref(desc), dims, off))
+    return tuple(dims), tuple(off)
+
+
+def cgrid_desc(ni: int, nj: int, nk: int, precision: int = FP64, naca=(0.0, 0.4, 0.12), wake: float = 20.0,
+               radius: float = 20.0, wall: float = 1e-4, span: float = 0.25, n_wake: int | None = None,
+               aoa_deg: float = 0.0) -> tuple[BlockDesc, CGrid]:
+    """Descriptor and parameters of a NACA 4-digit C-grid block (naca = (m, p, t) as chord fractions)."""
+    g = CGrid(naca[0], naca[1], naca[2], wake, radius, wall, span, n_wake if n_wake else ni // 4, 0)
+    d = BlockDesc()
+    _check(_lib.cfdx_geom_cgrid_desc(ni, nj, nk, precision, C.byref(g), math.radians(aoa_deg), C.byref(d)))
+    return d, g
+
+
+def cgrid_geometry(desc: BlockDesc, g: CGrid) -> tuple[np.ndarray, np.ndarray]:
+    """(xyz of the stored nodes, the 22 geometry arrays) of a C-grid block."""
+    dims, _ = padded(desc)
+    N = int(np.prod(dims))
+    xyz = np.zeros((N, 3))
+    _check(_lib.cfdx_geom_cgrid_coords(C.byref(desc), C.byref(g), _ptr(xyz)))
+    L = np.array([0.0, 0.0, g.span])
+    geo = np.zeros(22 * N)
+    _check(_lib.cfdx_geom_block_metrics(C.byref(desc), _ptr(xyz), _ptr(L), _ptr(geo)))
+    return xyz, geo
+
+
+def box_geometry(desc: BlockDesc, length=(1.0, 1.0, 1.0), amp: float = 0.0, phases=(0.0, 0.0, 0.0)):
+    """(xyz, geometry arrays) of a sinusoidally mapped box over the stored nodes of a descriptor."""
+    dims, _ = padded(desc)
+    N = int(np.prod(dims))
+    L = np.ascontiguousarray(length, dtype=np.float64)
+    ph = np.ascontiguousarray(phases, dtype=np.float64)
+    xyz = np.zeros((N, 3))
+    _check(_lib.cfdx_geom_box_coords(C.byref(desc), _ptr(L), amp, _ptr(ph), _ptr(xyz)))
+    geo = np.zeros(22 * N)
+    _check(_lib.cfdx_geom_block_metrics(C.byref(desc), _ptr(xyz), _ptr(L), _ptr(geo)))
+    return xyz, geo
+
+
+def owned_index(desc: BlockDesc) -> np.ndarray:
+    """Stored index of every owned node, in owned order."""
+    dims, off = padded(desc)
+    k, j, i = np.meshgrid(np.arange(desc.nk), np.arange(desc.nj), np.arange(desc.ni), indexing="ij")
+    return ((i + off[0]) + dims[0] * ((j + off[1]) + dims[1] * (k + off[2]))).reshape(-1)
+
+
+def freestream(n: int, mach: float, aoa_deg: float = 0.0) -> np.ndarray:
+    """Uniform free stream u = cos(aoa), v = sin(aoa), p = 1 / (1.4 M^2), T = 1 (Flow_Initialisation.cpp:35-48)."""
+    f = np.zeros(n, dtype=FLOW_DT)
+    a = math.radians(aoa_deg)
+    p = 1.0 / (1.4 * mach * mach)
+    f["u"], f["v"], f["w"], f["p"], f["t"] = math.cos(a), math.sin(a), 0.0, p, 1.0
+    f["rho"] = (1.4 * mach * mach) * (p / 1.0)
+    f["e"] = p / (0.4 * f["rho"])
+    f["a"] = np.sqrt(1.4 * p / f["rho"])
+    return f
+
+
 def cfl_dt(case: dict, cfl: float) -> float:
     """dt = cfl / max_nodes sum_d (|u . grad xi_d| + c |grad xi_d|) on the initial state (computed once)."""
     fl = case["FLOW"][1:]
@@ -409,21 +489,41 @@ class DropInSolver:
 # structured block (SoA, directional sweeps)
 # ---------------------------------------------------------------------------
 class Block:
-    """One periodic ni x nj x nk block resident in HBM (nodes in i-fastest order)."""
+    """One ni x nj x nk block resident in HBM (owned nodes in i-fastest order). Fully periodic by
+    default; a BlockDesc (e.g. from cgrid_desc) gives bounded directions with side conditions."""
 
-    def __init__(self, ni: int, nj: int, nk: int, precision: int = FP64, ctx: Ctx | None = None):
+    def __init__(self, ni: int, nj: int, nk: int, precision: int = FP64, ctx: Ctx | None = None,
+                 desc: BlockDesc | None = None):
         self.ctx = ctx or Ctx()
         self.precision = precision
-        self.desc = BlockDesc(ni, nj, nk, 7, precision)
+        if desc is None:
+            desc = BlockDesc(ni, nj, nk, 7, precision)
+        else:
+            desc.precision = precision
+            assert (desc.ni, desc.nj, desc.nk) == (ni, nj, nk)
+        self.desc = desc
         h = _vp()
         _check(_lib.cfdx_block_create(self.ctx.h, C.byref(self.desc), C.byref(h)))
         self.h = h
         self.n = ni * nj * nk
 
+    @classmethod
+    def of(cls, desc: BlockDesc, ctx: Ctx | None = None) -> "Block":
+        return cls(desc.ni, desc.nj, desc.nk, precision=desc.precision, ctx=ctx, desc=desc)
+
     @staticmethod
-    def bytes_for(ni: int, nj: int, nk: int, precision: int = FP64) -> int:
-        d = BlockDesc(ni, nj, nk, 7, precision)
+    def bytes_for(ni: int, nj: int, nk: int, precision: int = FP64, desc: BlockDesc | None = None) -> int:
+        d = desc if desc is not None else BlockDesc(ni, nj, nk, 7, precision)
+        if desc is not None:
+            d = BlockDesc.from_buffer_copy(desc)
+            d.precision = precision
         return int(_lib.cfdx_block_bytes(C.byref(d)))
+
+    def set_geometry_soa(self, geo: np.ndarray) -> None:
+        dims, _ = padded(self.desc)
+        geo = np.ascontiguousarray(geo, dtype=np.float64)
+        assert geo.size == 22 * int(np.prod(dims))
+        _check(_lib.cfdx_block_set_geometry_soa(self.h, _ptr(geo)))
 
     def close(self) -> None:
         if self.h:

@@ -56,11 +56,89 @@ namespace {
 
 size_t flux_size(int precision) { return precision == CFDX_MIXED ? sizeof(float) : sizeof(double); }
 
+// padded sizes and owned offsets: 3 ghost layers on each side of a bounded direction
+void padded_dims(const cfdx_block_desc *d, int dims[3], int off[3])
+{
+    const int n[3] = {d->ni, d->nj, d->nk};
+    for (int q = 0; q < 3; ++q) {
+        off[q] = ((d->periodic >> q) & 1) ? 0 : 3;
+        dims[q] = n[q] + 2 * off[q];
+    }
+}
+
+int64_t stored_nodes(const cfdx_block_desc *d)
+{
+    int dims[3], off[3];
+    padded_dims(d, dims, off);
+    return (int64_t)dims[0] * dims[1] * dims[2];
+}
+
 int64_t bytes_of(const cfdx_block_desc *d)
 {
-    const int64_t n = (int64_t)d->ni * d->nj * d->nk;
+    const int64_t n = stored_nodes(d);
     // 33 double arrays (geometry 22, U0 5, Us 5, mu 1) + 32 flux-type arrays (P 5, Fv 12, FH 15)
-    return n * (33 * 8 + 32 * (int64_t)flux_size(d->precision)) + 256;
+    // + one byte of loc code per node for a bounded block
+    const int64_t codes = d->periodic == 7 ? 0 : ((n + 255) / 256) * 256;
+    return n * (33 * 8 + 32 * (int64_t)flux_size(d->precision)) + codes + 256;
+}
+
+int check_desc(const cfdx_block_desc *d)
+{
+    if (d->ni < 3 || d->nj < 3 || d->nk < 3) return cfdx::fail(CFDX_E_INVALID, "block_create: each dimension must be >= 3");
+    if (d->periodic < 0 || d->periodic > 7) return cfdx::fail(CFDX_E_INVALID, "block_create: periodic is a 3-bit mask, got %d", d->periodic);
+    if (d->precision != CFDX_FP64 && d->precision != CFDX_MIXED)
+        return cfdx::fail(CFDX_E_INVALID, "block_create: precision %d is neither CFDX_FP64 nor CFDX_MIXED", d->precision);
+    const int n[3] = {d->ni, d->nj, d->nk};
+    for (int q = 0; q < 3; ++q) {
+        if ((d->periodic >> q) & 1) continue;
+        if (n[q] < 7) return cfdx::fail(CFDX_E_INVALID, "block_create: a bounded direction needs >= 7 nodes (direction %d has %d)", q, n[q]);
+        for (int h = 0; h < 2; ++h) {
+            const int bc = d->bc[2 * q + h];
+            if (bc < CFDX_BC_WALL || bc > CFDX_BC_WALL_CUT)
+                return cfdx::fail(CFDX_E_INVALID, "block_create: side %d of a bounded direction has no condition (%d)", 2 * q + h, bc);
+            if (bc == CFDX_BC_WALL_CUT && 2 * q + h != 2)
+                return cfdx::fail(CFDX_E_INVALID, "block_create: CFDX_BC_WALL_CUT is a j- side condition");
+        }
+    }
+    if (!(d->periodic & 2) && d->bc[2] == CFDX_BC_WALL_CUT &&
+        !(d->cut_i0 >= 0 && d->cut_i0 <= d->cut_i1 && d->cut_i1 < d->ni && d->cut_i0 + d->cut_i1 == d->ni - 1))
+        return cfdx::fail(CFDX_E_INVALID, "block_create: the wall range of a wake cut must be symmetric in i (cut_i0 + cut_i1 = ni - 1)");
+    const long long np = stored_nodes(d);
+    if (np >= (1LL << 29)) return cfdx::fail(CFDX_E_CAPACITY, "block_create: %lld stored nodes, the limit is 2^29 (32-bit byte offsets)", np);
+    return CFDX_OK;
+}
+
+// loc codes of a bounded block: 0 interior, 1 node on a side (boundary condition node), 100 ghost;
+// a ghost across a wake cut stands for its mirror node and takes the mirror's code
+std::vector<uint8_t> loc_codes(const cfdx_block_desc *d)
+{
+    int dims[3], off[3];
+    padded_dims(d, dims, off);
+    const int n[3] = {d->ni, d->nj, d->nk};
+    const bool cut = !(d->periodic & 2) && d->bc[2] == CFDX_BC_WALL_CUT;
+    auto wall_col = [&](int i) { return i >= d->cut_i0 && i <= d->cut_i1; };
+    std::vector<uint8_t> c((size_t)dims[0] * dims[1] * dims[2], 100);
+    auto at = [&](int i, int j, int k) {
+        return (size_t)(i + off[0]) + (size_t)dims[0] * ((size_t)(j + off[1]) + (size_t)dims[1] * (size_t)(k + off[2]));
+    };
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const int x[3] = {i, j, k};
+                bool side = false;
+                for (int q = 0; q < 3; ++q) {
+                    if (!off[q]) continue;
+                    if (x[q] == n[q] - 1) side = true;
+                    if (x[q] == 0 && !(q == 1 && cut && !wall_col(i))) side = true;
+                }
+                c[at(i, j, k)] = side ? 1 : 0;
+            }
+    if (cut)
+        for (int k = 0; k < n[2]; ++k)
+            for (int i = 0; i < n[0]; ++i)
+                if (!wall_col(i))
+                    for (int m = 1; m <= 3; ++m) c[at(i, -m, k)] = c[at(n[0] - 1 - i, m, k)];
+    return c;
 }
 
 int check_params(const cfdx_params *p)
@@ -83,6 +161,28 @@ void configure(cfdx_block *b, double mach, double del_t, const cfdx_params *p)
     b->last_mach = mach;
     b->configured = true;
     if (std::memcmp(&old, &b->B, sizeof old) != 0) ++b->version;
+}
+
+// values of the owned nodes (owned order) from an array over the stored nodes
+void gather_owned(const cfdx_block *b, const double *stored, double *owned)
+{
+    const int ni = b->desc.ni, nj = b->desc.nj, nk = b->desc.nk;
+    const int *off = b->B.off;
+    size_t o = 0;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i)
+                owned[o++] = stored[(size_t)(i + off[0]) + (size_t)b->B.pi * ((size_t)(j + off[1]) + (size_t)b->B.pj * (size_t)(k + off[2]))];
+}
+
+// side-condition extent of one block (largest side face of a bounded direction) and wake cut
+void bc_extent(const BlockDev &B, uint32_t &face, bool &cut)
+{
+    if (!B.code) return;
+    if (B.off[0]) face = std::max(face, (uint32_t)B.nj * B.nk);
+    if (B.off[1]) face = std::max(face, (uint32_t)B.ni * B.nk);
+    if (B.off[2]) face = std::max(face, (uint32_t)B.ni * B.nj);
+    if (B.off[1] && B.bc[2] == CFDX_BC_WALL_CUT) cut = true;
 }
 
 // single-block launch list (tile offsets 0 / total)
@@ -109,6 +209,7 @@ LaunchList single_list(cfdx_block *b)
     L.blks = b->dB;
     L.nblk = 1;
     L.tiles = b->tiles;
+    bc_extent(b->B, L.bc_face, L.any_cut);
     for (int kind = 0; kind < 4; ++kind) {
         L.prefix[kind] = b->dprefix + 2 * kind;
         L.total[kind] = cfdx::blk::tiles_of(b->B, kind, b->tiles);
@@ -190,6 +291,9 @@ int upload_batch(cfdx_batch *q)
     }
     q->L.grid_cap = q->grid_cap;
     q->L.tiles = q->tiles;
+    q->L.bc_face = 0;
+    q->L.any_cut = false;
+    for (size_t i = 0; i < nb; ++i) bc_extent(h[i], q->L.bc_face, q->L.any_cut);
     q->seen.resize(nb);
     for (size_t i = 0; i < nb; ++i) q->seen[i] = q->blocks[i]->version;
     q->dirty = false;
@@ -206,7 +310,17 @@ extern "C" int64_t cfdx_block_bytes(const cfdx_block_desc *d)
 {
     if (!d || d->ni < 1 || d->nj < 1 || d->nk < 1) return -1;
     if (d->precision != CFDX_FP64 && d->precision != CFDX_MIXED) return -1;
+    if (d->periodic < 0 || d->periodic > 7) return -1;
     return bytes_of(d);
+}
+
+extern "C" int cfdx_block_padded(const cfdx_block_desc *desc, int32_t dims[3], int32_t off[3])
+{
+    if (!desc || !dims || !off) return cfdx::fail(CFDX_E_INVALID, "block_padded: null argument");
+    int d[3], o[3];
+    padded_dims(desc, d, o);
+    for (int q = 0; q < 3; ++q) { dims[q] = d[q]; off[q] = o[q]; }
+    return CFDX_OK;
 }
 
 extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfdx_block **out)
@@ -214,12 +328,8 @@ extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfd
     CFDX_TRY(cfdx::enter(ctx));
     if (!desc || !out) return cfdx::fail(CFDX_E_INVALID, "block_create: null argument");
     *out = nullptr;
-    if (desc->ni < 3 || desc->nj < 3 || desc->nk < 3) return cfdx::fail(CFDX_E_INVALID, "block_create: each dimension must be >= 3");
-    if (desc->periodic != 7) return cfdx::fail(CFDX_E_UNSUPPORTED, "block_create: only fully periodic blocks (periodic=7)");
-    if (desc->precision != CFDX_FP64 && desc->precision != CFDX_MIXED)
-        return cfdx::fail(CFDX_E_INVALID, "block_create: precision %d is neither CFDX_FP64 nor CFDX_MIXED", desc->precision);
-    const long long n = (long long)desc->ni * desc->nj * desc->nk;
-    if (n >= (1LL << 29)) return cfdx::fail(CFDX_E_CAPACITY, "block_create: %lld nodes, the limit is 2^29 (32-bit byte offsets)", n);
+    CFDX_TRY(check_desc(desc));
+    const long long n = stored_nodes(desc);
     cfdx_block *b = new cfdx_block;
     b->ctx = ctx;
     cfdx::ctx_retain(ctx);
@@ -242,6 +352,17 @@ extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfd
     }
     BlockDev &B = b->B;
     B.ni = desc->ni; B.nj = desc->nj; B.nk = desc->nk; B.n = (uint32_t)n;
+    {
+        int dims[3];
+        padded_dims(desc, dims, B.off);
+        B.pi = dims[0];
+        B.pj = dims[1];
+        for (int q = 0; q < 6; ++q) B.bc[q] = desc->bc[q];
+        B.cut_i0 = desc->cut_i0;
+        B.cut_i1 = desc->cut_i1;
+        B.cos_aoa = std::cos(desc->aoa);
+        B.sin_aoa = std::sin(desc->aoa);
+    }
     char *p = (char *)b->mem;
     auto takeD = [&]() { double *r = (double *)p; p += n * sizeof(double); return r; };
     auto takeS = [&]() { void *r = p; p += n * flux_size(desc->precision); return r; };
@@ -256,6 +377,17 @@ extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfd
     for (auto &r : B.Fv) for (auto &a : r) a = takeS();
     for (auto &r : B.FH) for (auto &a : r) a = takeS();
     B.Lout = nullptr;
+    B.code = nullptr;
+    if (desc->periodic != 7) {
+        const std::vector<uint8_t> codes = loc_codes(desc);
+        B.code = (const uint8_t *)p;
+        e = hipMemcpyAsync((void *)B.code, codes.data(), codes.size(), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            cfdx_block_destroy(b);
+            return cfdx::hip_fail(e, "block_create codes");
+        }
+    }
     cfdx_params dp;
     cfdx_params_default(&dp);
     configure(b, 0.5, 0.0, &dp);
@@ -283,6 +415,8 @@ extern "C" int cfdx_block_destroy(cfdx_block *b)
 extern "C" int cfdx_block_set_geometry(cfdx_block *b, const double *det, const cfdx_metric *metric, const cfdx_deter *deter)
 {
     if (!b || !det || !metric || !deter) return cfdx::fail(CFDX_E_INVALID, "set_geometry: null argument");
+    if (b->desc.periodic != 7)
+        return cfdx::fail(CFDX_E_UNSUPPORTED, "set_geometry: a bounded block needs its ghost-node geometry (cfdx_block_set_geometry_soa)");
     CFDX_TRY(cfdx::enter(b->ctx));
     const int ni = b->desc.ni, nj = b->desc.nj, nk = b->desc.nk;
     const long long n = (long long)ni * nj * nk;
@@ -322,19 +456,36 @@ extern "C" int cfdx_block_set_geometry(cfdx_block *b, const double *det, const c
     return CFDX_OK;
 }
 
+extern "C" int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo)
+{
+    if (!b || !geo) return cfdx::fail(CFDX_E_INVALID, "set_geometry_soa: null argument");
+    CFDX_TRY(cfdx::enter(b->ctx));
+    for (size_t q = 0; q < (size_t)22 * b->B.n; ++q)
+        if (!std::isfinite(geo[q])) return cfdx::fail(CFDX_E_NONFINITE, "set_geometry_soa: value %zu is not finite", q);
+    CFDX_HIP(hipMemcpyAsync((void *)b->B.det, geo, (size_t)22 * b->B.n * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry_soa copy");
+    CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "set_geometry_soa sync");
+    return CFDX_OK;
+}
+
 extern "C" int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow)
 {
     if (!b || !flow) return cfdx::fail(CFDX_E_INVALID, "set_flow: null argument");
     CFDX_TRY(cfdx::enter(b->ctx));
-    const long long n = b->B.n;
+    const long long n = (long long)b->desc.ni * b->desc.nj * b->desc.nk;
     std::vector<double> h((size_t)5 * n);
     for (long long g = 0; g < n; ++g) {
         h[g] = flow[g].rho; h[n + g] = flow[g].u; h[2 * n + g] = flow[g].v; h[3 * n + g] = flow[g].w; h[4 * n + g] = flow[g].p;
     }
-    // stage the double primitives in Us (contiguous 5 arrays), then P and U0 = det * Q(P)
+    // stage the double primitives in Us (5 n_owned <= 5 n_stored doubles), then P and U0 = det * Q(P)
     CFDX_HIP(hipMemcpyAsync(b->B.Us[0], h.data(), h.size() * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_flow copy");
-    cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream);
+    cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream, b->B.Us[0]);
     CFDX_TRY(cfdx::check_launch("k_base"));
+    if (b->B.code) {
+        // side conditions of the initial state (boundary and ghost nodes, U0 included)
+        CFDX_TRY(upload_single(b));
+        cfdx::blk::boundary(single_list(b), b->desc.precision, 2, b->ctx->stream);
+        CFDX_TRY(cfdx::check_launch("boundary"));
+    }
     CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "set_flow sync");
     return CFDX_OK;
 }
@@ -343,16 +494,18 @@ extern "C" int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow)
 {
     if (!b || !flow) return cfdx::fail(CFDX_E_INVALID, "get_flow: null argument");
     CFDX_TRY(cfdx::enter(b->ctx));
-    const long long n = b->B.n;
+    const long long n = (long long)b->desc.ni * b->desc.nj * b->desc.nk;
     std::vector<double> h((size_t)6 * n);
     // the face-flux arrays are scratch between stages: 15 n flux-type values >= 5 n doubles
     double *tmp = (double *)b->B.FH[0][0];
     cfdx::blk::prims(b->B, b->desc.precision, b->ctx->stream, tmp);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), tmp, (size_t)5 * n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data() + 5 * n, b->B.mu, (size_t)n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
+    std::vector<double> mu(b->B.n);
+    if (e == hipSuccess) e = hipMemcpyAsync(mu.data(), b->B.mu, (size_t)b->B.n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
     if (e != hipSuccess) return cfdx::hip_fail(e, "get_flow");
+    gather_owned(b, mu.data(), h.data() + 5 * n);
     const double M = b->last_mach;
     for (long long g = 0; g < n; ++g) {
         cfdx_flow &f = flow[g];
@@ -372,12 +525,13 @@ extern "C" int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c)
         return cfdx::fail(CFDX_E_INVALID, "init_case: lengths and Mach must be positive");
     if (c->init < CFDX_INIT_FREESTREAM || c->init > CFDX_INIT_TGV)
         return cfdx::fail(CFDX_E_INVALID, "init_case: unknown initial state %d", c->init);
+    if (b->desc.periodic != 7) return cfdx::fail(CFDX_E_UNSUPPORTED, "init_case: the sinusoidal map is for fully periodic blocks");
     CFDX_TRY(cfdx::enter(b->ctx));
     cfdx::blk::init_geometry(b->B, c->length, c->amp, c->phase, b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_geometry"));
     cfdx::blk::init_state(b->B, c->length, c->amp, c->phase, c->init, c->mach, c->beta, c->center[0], c->center[1], b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_state"));
-    cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream);
+    cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream, b->B.Us[0]);
     CFDX_TRY(cfdx::check_launch("k_base"));
     b->last_mach = c->mach;
     return CFDX_OK;
@@ -438,9 +592,12 @@ extern "C" int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *
     int rc = upload_single(b);
     if (rc == CFDX_OK) rc = run_stage(b, 3);
     if (rc == CFDX_OK) {
-        hipError_t e = hipMemcpyAsync(L_host, Ld, (size_t)5 * n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
+        std::vector<double> hs((size_t)5 * n);
+        hipError_t e = hipMemcpyAsync(hs.data(), Ld, (size_t)5 * n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
         if (e != hipSuccess) rc = cfdx::hip_fail(e, "block_rhs copy");
+        const size_t no = (size_t)b->desc.ni * b->desc.nj * b->desc.nk;
+        for (int c = 0; c < 5 && rc == CFDX_OK; ++c) gather_owned(b, hs.data() + (size_t)c * n, L_host + c * no);
     }
     b->B.Lout = nullptr;
     ++b->version;

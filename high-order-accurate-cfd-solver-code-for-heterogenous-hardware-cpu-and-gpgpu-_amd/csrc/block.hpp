@@ -32,8 +32,14 @@ struct TileCfg {
 // Device-side view of one block: sizes, array pointers and per-block run parameters.
 // P, Fv and FH hold the flux type (double or float); everything else is double.
 struct BlockDev {
-    int ni, nj, nk;
-    uint32_t n;
+    int ni, nj, nk;             // owned nodes
+    uint32_t n;                 // stored (padded) nodes: the length of every array
+    int pi, pj;                 // padded sizes along i and j (index strides)
+    int off[3];                 // ghost layers before the first owned node along i, j, k (3, or 0 if periodic)
+    const uint8_t *code;        // bounded blocks: per stored node 0 interior, 1 boundary node, 100 ghost
+    int bc[6];                  // side conditions (CFDX_BC_*) of i-, i+, j-, j+, k-, k+
+    int cut_i0, cut_i1;         // wall range of a CFDX_BC_WALL_CUT side
+    double cos_aoa, sin_aoa;    // far-field flow direction
     uint32_t tile0[4];          // first global tile of this block: node tiles, face tiles x, y, z
     const double *det;          // det at the node
     const double *m[9];         // node metrics zeta_x .. xi_z
@@ -60,15 +66,21 @@ struct LaunchList {
     uint32_t total[4] = {};
     uint32_t grid_cap = 0;            // 0: one workgroup per tile; else persistent grid of this size
     TileCfg tiles;
+    uint32_t bc_face = 0;             // largest side face (owned nodes) of the bounded blocks; 0 if none
+    bool any_cut = false;             // some block has a wake cut
 };
 
 // kernels (block_kernels.hip)
 // ev (optional): 6 events recorded before the viscous kernel, after it, after each face sweep
 // and after the update
 void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev);
-void base(const BlockDev &B, int precision, hipStream_t st);
+// P and U0 of the owned nodes from double primitives staged component-major in owned order
+void base(const BlockDev &B, int precision, hipStream_t st, const double *stage);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
 uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg);
+// side conditions of the bounded blocks of a list (mode 0..2 after that RK stage, mode 2 also
+// refreshing U0), side by side in the order i-, i+, j-, j+, k-, k+; stage() runs it after the update
+void boundary(const LaunchList &L, int precision, int mode, hipStream_t st);
 // out[0..2] (device) = maxima of count per-wave monitor triples
 void monitor_reduce(const double *mon, uint32_t count, double *out, hipStream_t st);
 

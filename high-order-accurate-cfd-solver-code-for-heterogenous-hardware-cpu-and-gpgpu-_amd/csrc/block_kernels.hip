@@ -41,17 +41,33 @@ __device__ __forceinline__ void st(T *base, uint32_t i, T v)
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// stored index of owned coordinates (i, j, k); a bounded direction may go 3 nodes beyond the
+// owned range into its ghost layers
 __device__ __forceinline__ uint32_t node_index(const BlockDev &B, int i, int j, int k)
 {
-    return (uint32_t)i + (uint32_t)B.ni * ((uint32_t)j + (uint32_t)B.nj * (uint32_t)k);
+    return (uint32_t)(i + B.off[0]) + (uint32_t)B.pi * ((uint32_t)(j + B.off[1]) + (uint32_t)B.pj * (uint32_t)(k + B.off[2]));
+}
+
+// owned coordinate c + o along a direction of n owned nodes: wrapped if the direction is periodic
+__device__ __forceinline__ int along_shift(const BlockDev &B, int d, int c, int o, int n)
+{
+    return B.off[d] ? c + o : wrapi(c + o, n);
 }
 
 __device__ __forceinline__ uint32_t shifted(const BlockDev &B, int i, int j, int k, int d, int o)
 {
-    if (d == 0) i = wrapi(i + o, B.ni);
-    else if (d == 1) j = wrapi(j + o, B.nj);
-    else k = wrapi(k + o, B.nk);
+    if (d == 0) i = along_shift(B, 0, i, o, B.ni);
+    else if (d == 1) j = along_shift(B, 1, j, o, B.nj);
+    else k = along_shift(B, 2, k, o, B.nk);
     return node_index(B, i, j, k);
+}
+
+// number of +1/2 faces a sweep along d computes per line: one more than the owned nodes in a
+// bounded direction (the face between the first owned node and its ghost)
+__device__ __host__ __forceinline__ int faces_along(const BlockDev &B, int d)
+{
+    const int n = d == 0 ? B.ni : d == 1 ? B.nj : B.nk;
+    return B.off[d] ? n + 1 : n;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each
@@ -77,7 +93,7 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ pref
 // ---------------------------------------------------------------------------
 // viscous node kernel (Viscous.cl:285-631 on a periodic block: central 4th order)
 // ---------------------------------------------------------------------------
-template <typename S, int TX, int TY>
+template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                            uint32_t nblk, uint32_t total, int mode)
 {
@@ -104,12 +120,27 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
             const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
             const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
             double um1[5], up1[5];
+            if (BND && B.code) {
+                // bounded block: the loc-selected one-sided forms of Viscous.cl:285-435 near the sides
+                const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
+                const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
+                          LP2 = ld(B.code, p2);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                const double *u = Uin[c];
-                um1[c] = ld(u, m1);
-                up1[c] = ld(u, p1);
-                dU[c][d] = (1.0 / 12.0) * (8.0 * (up1[c] - um1[c]) - (ld(u, p2) - ld(u, m2)));
+                for (int c = 0; c < 5; ++c) {
+                    const double *u = Uin[c];
+                    um1[c] = ld(u, m1);
+                    up1[c] = ld(u, p1);
+                    const double f[7] = {ld(u, m3), ld(u, m2), um1[c], U[c], up1[c], ld(u, p2), ld(u, p3)};
+                    dU[c][d] = loc_derivative(f, LI, LM2, LM1, LP1, LP2, false);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    const double *u = Uin[c];
+                    um1[c] = ld(u, m1);
+                    up1[c] = ld(u, p1);
+                    dU[c][d] = (1.0 / 12.0) * (8.0 * (up1[c] - um1[c]) - (ld(u, p2) - ld(u, m2)));
+                }
             }
             // MTS scale-similarity term of momentum component d+1 along d (Viscous.cl:493-495)
             const double a = (U[1 + d] * iU0) - ((0.25 * um1[1 + d] + 0.5 * U[1 + d] + 0.25 * up1[1 + d]) / (0.25 * um1[0] + 0.5 * U[0] + 0.25 * up1[0]));
@@ -174,7 +205,10 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
         const uint32_t lt = t - B.tile0[1 + DIR];
         const int n_along = DIR == 0 ? B.ni : DIR == 1 ? B.nj : B.nk;
         const int n_across = DIR == 0 ? B.nj : B.ni;
-        const uint32_t nta = (n_across + TA - 1) / TA, ntb = (n_along + TB - 1) / TB;
+        // face f of a line is the +1/2 face of along-coordinate f - sh (sh = 1 in a bounded
+        // direction: the first face lies between the first owned node and its ghost)
+        const int nfa = faces_along(B, DIR), sh = B.off[DIR] ? 1 : 0;
+        const uint32_t nta = (n_across + TA - 1) / TA, ntb = (nfa + TB - 1) / TB;
         const uint32_t nfix = DIR == 2 ? B.nj : B.nk;
         int b0, a0, fixed;
         if (DIR == 0) { b0 = (lt % ntb) * TB; a0 = ((lt / ntb) % nta) * TA; fixed = lt / (ntb * nta); }
@@ -187,7 +221,10 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
             const int s = DIR == 0 ? p % NP : p / TA;
             const int a = DIR == 0 ? p / NP : p % TA;
             if (a0 + a >= n_across) continue;
-            const int along = wrapi(b0 + s - 2, n_along);
+            const int along = along_shift(B, DIR, b0 - sh, s - 2, n_along);
+            // a bounded line ends 3 ghost nodes past the last owned node; the points of a partial
+            // last tile beyond it feed no computed face
+            if (sh && along > n_along + 2) continue;
             const uint32_t x = tile_node<DIR>(B, along, a0 + a, fixed);
             const S rho = ld(P0, x), u = ld(P1, x), v = ld(P2, x), w = ld(P3, x), pr = ld(P4, x);
             S q[5], f[5];
@@ -204,14 +241,14 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 
         const int b = DIR == 0 ? tid % TB : tid / TA;
         const int a = DIR == 0 ? tid / TB : tid % TA;
-        if (b0 + b < n_along && a0 + a < n_across) {
+        if (b0 + b < nfa && a0 + a < n_across) {
             S q[6][5];
 #pragma unroll
             for (int m = 0; m < 6; ++m)
 #pragma unroll
                 for (int c = 0; c < 5; ++c) q[m][c] = sq[lds_at<DIR, TA, TB>(c, b + m, a)];
-            const uint32_t gl = tile_node<DIR>(B, b0 + b, a0 + a, fixed);
-            const uint32_t gr = tile_node<DIR>(B, wrapi(b0 + b + 1, n_along), a0 + a, fixed);
+            const uint32_t gl = tile_node<DIR>(B, b0 + b - sh, a0 + a, fixed);
+            const uint32_t gr = tile_node<DIR>(B, along_shift(B, DIR, b0 + b - sh, 1, n_along), a0 + a, fixed);
             S roe[5];
             {
                 const S ra = ld(P0, gl), ua = ld(P1, gl), va = ld(P2, gl), wa = ld(P3, gl), pa = ld(P4, gl);
@@ -235,7 +272,7 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 // update kernel: L and the SSP-RK3 stage (Weno.cl:3060-3135), in double.
 //   mode 0..2: RK stage j; mode 3: write L to the block's Lout only
 // ---------------------------------------------------------------------------
-template <typename S, int TX, int TY>
+template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                    uint32_t nblk, uint32_t total, int mode)
 {
@@ -266,10 +303,24 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
                 for (int d = 0; d < 3; ++d) {
                     const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
                     const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+                    if (BND && B.code) {
+                        // bounded block: the loc-selected forms of Weno.cl:2732-2856
+                        const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
+                        const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
+                                  LP2 = ld(B.code, p2);
 #pragma unroll
-                    for (int c = 1; c < 5; ++c) {
-                        const S *f = (const S *)B.Fv[d][c - 1];
-                        L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
+                        for (int c = 1; c < 5; ++c) {
+                            const S *fv = (const S *)B.Fv[d][c - 1];
+                            const double f[7] = {(double)ld(fv, m3), (double)ld(fv, m2), (double)ld(fv, m1), (double)ld(fv, g),
+                                                 (double)ld(fv, p1), (double)ld(fv, p2), (double)ld(fv, p3)};
+                            L[c] += loc_derivative(f, LI, LM2, LM1, LP1, LP2, true);
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 1; c < 5; ++c) {
+                            const S *f = (const S *)B.Fv[d][c - 1];
+                            L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
+                        }
                     }
                 }
             }
@@ -349,13 +400,25 @@ __global__ void __launch_bounds__(256) k_mon_reduce(const double *__restrict__ m
     if (threadIdx.x < 3) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// set_flow: double primitives staged in Us -> P (type S) and U0 = det * Q(P) in double
-template <typename S>
-__global__ void k_base(BlockDev B)
+// owned node o (i-fastest over ni x nj x nk) -> stored index
+__device__ __forceinline__ uint32_t owned_node(const BlockDev &B, uint32_t o)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= B.n) return;
-    const double rho = ld(B.Us[0], g), u = ld(B.Us[1], g), v = ld(B.Us[2], g), w = ld(B.Us[3], g), p = ld(B.Us[4], g);
+    const int i = (int)(o % (uint32_t)B.ni), j = (int)((o / (uint32_t)B.ni) % (uint32_t)B.nj), k = (int)(o / ((uint32_t)B.ni * B.nj));
+    return node_index(B, i, j, k);
+}
+
+__device__ __forceinline__ uint32_t owned_count(const BlockDev &B) { return (uint32_t)B.ni * B.nj * B.nk; }
+
+// set_flow: double primitives staged component-major in owned order (stage[c * n_owned + o]) ->
+// P (type S) and U0 = det * Q(P) in double
+template <typename S>
+__global__ void k_base(BlockDev B, const double *__restrict__ stage)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x, no = owned_count(B);
+    if (o >= no) return;
+    const uint32_t g = owned_node(B, o);
+    const double rho = stage[o], u = stage[no + o], v = stage[2 * (size_t)no + o], w = stage[3 * (size_t)no + o],
+                 p = stage[4 * (size_t)no + o];
     double q[5];
     prim_to_q(rho, u, v, w, p, q);
     const double det = ld(B.det, g);
@@ -364,25 +427,158 @@ __global__ void k_base(BlockDev B)
     st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
 }
 
-// get_flow of the mixed form: primitives from the double RK base U0 / det
+// get_flow of the mixed form: primitives from the double RK base U0 / det, owned order
 __global__ void k_prims(BlockDev B, double *__restrict__ out)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= B.n) return;
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x, no = owned_count(B);
+    if (o >= no) return;
+    const uint32_t g = owned_node(B, o);
     const double idet = 1.0 / ld(B.det, g);
     const double rho = idet * ld(B.U0[0], g), ir = 1.0 / rho;
     const double u = (idet * ld(B.U0[1], g)) * ir, v = (idet * ld(B.U0[2], g)) * ir, w = (idet * ld(B.U0[3], g)) * ir;
     const double e = (idet * ld(B.U0[4], g)) * ir - 0.5 * (u * u + v * v + w * w);
-    out[g] = rho; out[B.n + g] = u; out[2 * (size_t)B.n + g] = v; out[3 * (size_t)B.n + g] = w; out[4 * (size_t)B.n + g] = e * rho * 0.4;
+    out[o] = rho; out[no + o] = u; out[2 * (size_t)no + o] = v; out[3 * (size_t)no + o] = w; out[4 * (size_t)no + o] = e * rho * 0.4;
 }
 
-// get_flow of the plain form: the primitives of the last stage
+// get_flow of the plain form: the primitives of the last stage, owned order
 __global__ void k_copy_prims(BlockDev B, double *__restrict__ out)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= B.n) return;
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x, no = owned_count(B);
+    if (o >= no) return;
+    const uint32_t g = owned_node(B, o);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) out[(size_t)c * B.n + g] = ld((const double *)B.P[c], g);
+    for (int c = 0; c < 5; ++c) out[(size_t)c * no + o] = ld((const double *)B.P[c], g);
+}
+
+// ---------------------------------------------------------------------------
+// side conditions of bounded blocks (Flow_Initialisation.cpp face rules; the C-grid wake cut)
+// ---------------------------------------------------------------------------
+template <typename S>
+__device__ __forceinline__ void get_prim(const BlockDev &B, uint32_t g, double (&w)[5])
+{
+#pragma unroll
+    for (int c = 0; c < 5; ++c) w[c] = (double)ld((const S *)B.P[c], g);
+}
+
+// primitives of a boundary or ghost node; its RK base (mode 2) or stage vector (modes 0, 1)
+// follows from them, as U_Q rebuilds U from FLOW (ghost stage vectors are not defined by the
+// reference at stages 1 and 2; here they are det * Q of the side condition's state)
+template <typename S>
+__device__ __forceinline__ void put_node(const BlockDev &B, uint32_t g, const double (&w)[5], int mode, bool ghost)
+{
+#pragma unroll
+    for (int c = 0; c < 5; ++c) st((S *)B.P[c], g, S(w[c]));
+    if (mode != 2 && !ghost) return;
+    double r[5];
+    get_prim<S>(B, g, r);
+    double q[5];
+    prim_to_q(r[0], r[1], r[2], r[3], r[4], q);
+    const double det = ld(B.det, g);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) st(mode == 2 ? B.U0[c] : B.Us[c], g, det * q[c]);
+}
+
+// the node across a wake cut into a ghost or duplicate node: primitives copied, the stage vector
+// and RK base carry the same Q with the destination's own det (U = det Q)
+__device__ __forceinline__ void copy_node(const BlockDev &B, uint32_t dst, uint32_t src, int ps)
+{
+    const double ratio = ld(B.det, dst) / ld(B.det, src);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        if (ps == 8) st((double *)B.P[c], dst, ld((const double *)B.P[c], src));
+        else st((float *)B.P[c], dst, ld((const float *)B.P[c], src));
+        st(B.Us[c], dst, ld(B.Us[c], src) * ratio);
+        st(B.U0[c], dst, ld(B.U0[c], src) * ratio);
+    }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_bc(const BlockDev *__restrict__ blks, int side, int mode)
+{
+    const BlockDev &B = blks[blockIdx.y];
+    const int d = side >> 1, hi = side & 1;
+    if (!B.off[d]) return;
+    const int na = d == 0 ? B.nj : B.ni, nb = d == 2 ? B.nj : B.nk;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint32_t)na * (uint32_t)nb) return;
+    const int a = (int)(t % (uint32_t)na), bq = (int)(t / (uint32_t)na);
+    const int n = d == 0 ? B.ni : d == 1 ? B.nj : B.nk;
+    auto at = [&](int c) -> uint32_t {
+        if (d == 0) return node_index(B, c, a, bq);
+        if (d == 1) return node_index(B, a, c, bq);
+        return node_index(B, a, bq, c);
+    };
+    const int c0 = hi ? n - 1 : 0, in = hi ? -1 : 1;   // boundary node; step into the block
+    int kind = B.bc[side];
+    if (kind == CFDX_BC_WALL_CUT) {
+        if (a >= B.cut_i0 && a <= B.cut_i1) {
+            kind = CFDX_BC_WALL;
+        } else {
+            // wake cut (j- side, a = i): node (i, -m) is node (ni-1-i, m); row 0 of the lower half
+            // copies the upper half
+            const int im = B.ni - 1 - a;
+            for (int m = 1; m <= 3; ++m) copy_node(B, node_index(B, a, -m, bq), node_index(B, im, m, bq), (int)sizeof(S));
+            if (a < im) copy_node(B, node_index(B, a, 0, bq), node_index(B, im, 0, bq), (int)sizeof(S));
+            return;
+        }
+    }
+    const double M2 = 1.4 * B.mach * B.mach;
+    double w1[5], w0[5];
+    get_prim<S>(B, at(c0 + in), w1);
+    if (kind == CFDX_BC_WALL) {
+        const double t1 = M2 * (w1[4] / w1[0]);
+        w0[0] = M2 * (w1[4] / t1); w0[1] = 0.0; w0[2] = 0.0; w0[3] = 0.0; w0[4] = w1[4];
+        put_node<S>(B, at(c0), w0, mode, false);
+        for (int m = 1; m <= 3; ++m) {
+            double wm[5];
+            get_prim<S>(B, at(c0 + in * m), wm);
+            const double wg[5] = {wm[0], -wm[1], -wm[2], -wm[3], wm[4]};
+            put_node<S>(B, at(c0 - in * m), wg, mode, true);
+        }
+        return;
+    }
+    if (kind == CFDX_BC_EXTRAP) {
+        put_node<S>(B, at(c0), w1, mode, false);
+        for (int m = 1; m <= 3; ++m) {
+            double wm[5];
+            get_prim<S>(B, at(c0 + in * m), wm);
+            put_node<S>(B, at(c0 - in * m), wm, mode, true);
+        }
+        return;
+    }
+    if (kind == CFDX_BC_FARFIELD) {
+        const double p = 1.0 / M2;
+        w0[0] = M2 * (p / 1.0); w0[1] = B.cos_aoa; w0[2] = B.sin_aoa; w0[3] = 0.0; w0[4] = p;
+    } else {   // CFDX_BC_OUTFLOW
+        const double t1 = M2 * (w1[4] / w1[0]);
+        w0[0] = M2 * (w1[4] / t1); w0[1] = w1[1]; w0[2] = w1[2]; w0[3] = w1[3]; w0[4] = w1[4];
+    }
+    put_node<S>(B, at(c0), w0, mode, false);
+    for (int m = 1; m <= 3; ++m) put_node<S>(B, at(c0 - in * m), w0, mode, true);
+}
+
+// viscous node fluxes of the ghost rows across a wake cut: the mirror node's, with the signs of
+// the i and j directions reversed (zeta and eta run the other way on the other side of the cut)
+template <typename S>
+__global__ void __launch_bounds__(256) k_cut_viscous(const BlockDev *__restrict__ blks)
+{
+    const BlockDev &B = blks[blockIdx.y];
+    if (!B.off[1] || B.bc[2] != CFDX_BC_WALL_CUT || !B.viscous) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint32_t)B.ni * (uint32_t)B.nk) return;
+    const int i = (int)(t % (uint32_t)B.ni), k = (int)(t / (uint32_t)B.ni);
+    if (i >= B.cut_i0 && i <= B.cut_i1) return;
+    const int im = B.ni - 1 - i;
+    for (int m = 1; m <= 3; ++m) {
+        const uint32_t dst = node_index(B, i, -m, k), src = node_index(B, im, m, k);
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const S v = ld((const S *)B.Fv[dd][c], src);
+                st((S *)B.Fv[dd][c], dst, dd == 2 ? v : -v);
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -400,12 +596,19 @@ void launch_face_jk(const LaunchList &L, dim3 g, hipStream_t st)
     hipLaunchKernelGGL((k_face<S, DIR, FACE_JK_TILES[V][0], FACE_JK_TILES[V][1]>), g, dim3(256), 0, st, L.blks,
                        L.prefix[1 + DIR], L.nblk, L.total[1 + DIR]);
 }
+template <typename S, int V, bool BND>
+void launch_node_b(const LaunchList &L, dim3 g, hipStream_t st, bool visc_kernel, int mode)
+{
+    constexpr int TX = NODE_TILES[V][0], TY = NODE_TILES[V][1];
+    if (visc_kernel) hipLaunchKernelGGL((k_visc<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    else hipLaunchKernelGGL((k_update<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+}
+// lists without bounded blocks run the node kernels without the loc-code branches
 template <typename S, int V>
 void launch_node(const LaunchList &L, dim3 g, hipStream_t st, bool visc_kernel, int mode)
 {
-    constexpr int TX = NODE_TILES[V][0], TY = NODE_TILES[V][1];
-    if (visc_kernel) hipLaunchKernelGGL((k_visc<S, TX, TY>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
-    else hipLaunchKernelGGL((k_update<S, TX, TY>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    if (L.bc_face) launch_node_b<S, V, true>(L, g, st, visc_kernel, mode);
+    else launch_node_b<S, V, false>(L, g, st, visc_kernel, mode);
 }
 
 template <typename S>
@@ -437,6 +640,8 @@ void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, 
     auto grid = [&](uint32_t total) { return dim3(L.grid_cap ? std::min(total, L.grid_cap) : total); };
     rec(0);
     if (any_visc) node_kernel<S>(L, grid(L.total[0]), st, true, mode);
+    if (any_visc && L.any_cut)
+        hipLaunchKernelGGL(k_cut_viscous<S>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks);
     rec(1);
     switch (L.tiles.face[0]) {
     case 1: launch_face_i<S, 1>(L, grid(L.total[1]), st); break;
@@ -449,6 +654,9 @@ void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, 
     face_jk<S, 2>(L, grid(L.total[3]), st);
     rec(4);
     node_kernel<S>(L, grid(L.total[0]), st, false, mode);
+    if (L.bc_face && mode <= 2)
+        for (int side = 0; side < 6; ++side)
+            hipLaunchKernelGGL(k_bc<S>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks, side, mode);
     rec(5);
 }
 
@@ -458,16 +666,27 @@ void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool an
     else launch_stage<double>(L, mode, st, any_visc, ev);
 }
 
-void base(const BlockDev &B, int precision, hipStream_t st)
+void base(const BlockDev &B, int precision, hipStream_t st, const double *stage)
 {
-    const unsigned nb = (unsigned)((B.n + 255) / 256);
-    if (precision == CFDX_MIXED) hipLaunchKernelGGL(k_base<float>, dim3(nb), dim3(256), 0, st, B);
-    else hipLaunchKernelGGL(k_base<double>, dim3(nb), dim3(256), 0, st, B);
+    const unsigned nb = (unsigned)(((uint32_t)B.ni * B.nj * B.nk + 255) / 256);
+    if (precision == CFDX_MIXED) hipLaunchKernelGGL(k_base<float>, dim3(nb), dim3(256), 0, st, B, stage);
+    else hipLaunchKernelGGL(k_base<double>, dim3(nb), dim3(256), 0, st, B, stage);
+}
+
+void boundary(const LaunchList &L, int precision, int mode, hipStream_t st)
+{
+    if (!L.bc_face) return;
+    for (int side = 0; side < 6; ++side) {
+        if (precision == CFDX_MIXED)
+            hipLaunchKernelGGL(k_bc<float>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks, side, mode);
+        else
+            hipLaunchKernelGGL(k_bc<double>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks, side, mode);
+    }
 }
 
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
 {
-    const unsigned nb = (unsigned)((B.n + 255) / 256);
+    const unsigned nb = (unsigned)(((uint32_t)B.ni * B.nj * B.nk + 255) / 256);
     if (precision == CFDX_MIXED) hipLaunchKernelGGL(k_prims, dim3(nb), dim3(256), 0, st, B, out);
     else hipLaunchKernelGGL(k_copy_prims, dim3(nb), dim3(256), 0, st, B, out);
 }
@@ -486,9 +705,9 @@ uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg)
     auto f = [&](const int (&t)[2], int along, int across, int fix) {
         return (uint32_t)(((across + t[0] - 1) / t[0]) * ((along + t[1] - 1) / t[1]) * fix);
     };
-    if (kind == 1) return f(FACE_I_TILES[cfg.face[0]], B.ni, B.nj, B.nk);
-    if (kind == 2) return f(FACE_JK_TILES[cfg.face[1]], B.nj, B.ni, B.nk);
-    return f(FACE_JK_TILES[cfg.face[2]], B.nk, B.ni, B.nj);
+    if (kind == 1) return f(FACE_I_TILES[cfg.face[0]], faces_along(B, 0), B.nj, B.nk);
+    if (kind == 2) return f(FACE_JK_TILES[cfg.face[1]], faces_along(B, 1), B.ni, B.nk);
+    return f(FACE_JK_TILES[cfg.face[2]], faces_along(B, 2), B.ni, B.nj);
 }
 
 }  // namespace blk

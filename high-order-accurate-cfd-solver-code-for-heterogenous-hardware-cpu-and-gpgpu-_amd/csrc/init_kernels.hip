@@ -146,7 +146,10 @@ __global__ void __launch_bounds__(RED) k_cfl_part(BlockDev B, double *part)
 {
     __shared__ double sh[RED];
     double r = 0.0;
-    for (uint32_t g = blockIdx.x * RED + threadIdx.x; g < B.n; g += gridDim.x * RED) {
+    const uint32_t no = (uint32_t)B.ni * B.nj * B.nk;
+    for (uint32_t o = blockIdx.x * RED + threadIdx.x; o < no; o += gridDim.x * RED) {
+        const uint32_t i = o % (uint32_t)B.ni, j = (o / (uint32_t)B.ni) % (uint32_t)B.nj, k = o / ((uint32_t)B.ni * B.nj);
+        const uint32_t g = (i + B.off[0]) + (uint32_t)B.pi * ((j + B.off[1]) + (uint32_t)B.pj * (k + B.off[2]));
         const double idet = 1.0 / B.det[g];
         const double rho = idet * B.U0[0][g], ir = 1.0 / rho;
         const double u = (idet * B.U0[1][g]) * ir, v = (idet * B.U0[2][g]) * ir, w = (idet * B.U0[3][g]) * ir;
@@ -190,7 +193,7 @@ void init_state(const BlockDev &B, const double L[3], double amp, const double p
     hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, st, B, a);
 }
 
-int cfl_parts(const BlockDev &B) { return (int)std::min<uint32_t>(1024u, (B.n + RED - 1) / RED); }
+int cfl_parts(const BlockDev &B) { return (int)std::min<uint32_t>(1024u, ((uint32_t)B.ni * B.nj * B.nk + RED - 1) / RED); }
 
 void cfl_radius(const BlockDev &B, double *part, hipStream_t st)
 {

@@ -145,19 +145,48 @@ int cfdx_weno(cfdx_ctx *ctx, int j, double Mach, double del_t, const cfdx_deter 
 /* ------------------------------------------------------------------------ */
 typedef struct cfdx_block cfdx_block;
 
+/* Side conditions of the directions that are not periodic (Flow_Initialisation.cpp face rules):
+ *   WALL      no-slip: u = v = w = 0, p and T of the first interior node, rho = 1.4 M^2 p / T; ghost
+ *             layer m mirrors interior node m with the velocity reversed (:952-988)
+ *   FARFIELD  free stream u = cos(aoa), v = sin(aoa), w = 0, p = 1 / (1.4 M^2), T = 1; ghosts copy
+ *             the boundary node (:384-418 with the inflow turned by aoa)
+ *   OUTFLOW   u, v, w, p, T of the first interior node, rho = 1.4 M^2 p / T; ghosts copy the
+ *             boundary node (:91-136)
+ *   EXTRAP    the boundary node copies the first interior node; ghost layer m copies interior node
+ *             m (:640-700)
+ *   WALL_CUT  (j- side only) WALL for cut_i0 <= i <= cut_i1; elsewhere the C-grid wake cut: node
+ *             (i, -m) is node (ni-1-i, m), and row j = 0 of the lower half copies the upper half.
+ * Bounded directions carry 3 ghost layers on each side; the Mach number is the block's (Weno) Mach. */
+enum { CFDX_BC_WALL = 1, CFDX_BC_FARFIELD = 2, CFDX_BC_OUTFLOW = 3, CFDX_BC_EXTRAP = 4, CFDX_BC_WALL_CUT = 5 };
+
 typedef struct {
-    int32_t ni, nj, nk;   /* nodes along i (zeta), j (eta), k (xi); node (i,j,k) is i + ni*(j + nj*k) */
-    int32_t periodic;     /* must be 7 (periodic in i, j and k) */
+    int32_t ni, nj, nk;   /* owned nodes along i (zeta), j (eta), k (xi); owned node (i,j,k) is i + ni*(j + nj*k) */
+    int32_t periodic;     /* bit 0: i, bit 1: j, bit 2: k periodic; 7 = fully periodic */
     int32_t precision;    /* CFDX_FP64, or CFDX_MIXED (float fluxes, double RK state and metrics) */
+    int32_t bc[6];        /* CFDX_BC_* of the sides i-, i+, j-, j+, k-, k+ (periodic directions: ignored) */
+    int32_t cut_i0, cut_i1;  /* wall range of a CFDX_BC_WALL_CUT side */
+    double aoa;           /* far-field flow angle in the x-y plane, radians */
 } cfdx_block_desc;
+
+/* Stored (padded) sizes of a block: owned nodes plus 3 ghost layers on each side of every bounded
+ * direction; padded node (i, j, k) is i + dims[0]*(j + dims[1]*k) and owned node (i, j, k) sits at
+ * padded (i + off[0], j + off[1], k + off[2]). */
+int cfdx_block_padded(const cfdx_block_desc *desc, int32_t dims[3], int32_t off[3]);
 
 /* Bytes of device memory a block of this shape holds. */
 int64_t cfdx_block_bytes(const cfdx_block_desc *desc);
 int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfdx_block **out);
 int cfdx_block_destroy(cfdx_block *b);
 
-/* Host arrays in node order (n = ni*nj*nk entries, reference AoS layouts). */
+/* Host arrays in node order (n = ni*nj*nk entries, reference AoS layouts; fully periodic blocks). */
 int cfdx_block_set_geometry(cfdx_block *b, const double *det, const cfdx_metric *metric, const cfdx_deter *deter);
+/* Geometry of any block as 22 arrays of padded length, one after the other: det, the 9 node
+ * metrics (zeta_x .. xi_z), the +1/2-face metric rows of i (zeta_?), j (eta_?) and k (xi_?) at each
+ * node's +1/2 face, and the three +1/2-face dets. Ghost nodes need their values too. */
+int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo);
+/* Flow of the owned nodes (ni*nj*nk entries in owned order). set_flow applies the side conditions
+ * of a bounded block (boundary and ghost nodes), as the reference does at start-up, with the Mach
+ * number of the last cfdx_block_configure / cfdx_block_step (0.5 before either). */
 int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow);
 int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow);
 
@@ -241,6 +270,36 @@ int cfdx_batch_set_grid(cfdx_batch *q, uint32_t grid_cap);
 int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph);
 /* Tile variants of the batch's launches (the blocks' own settings apply to single-block steps). */
 int cfdx_batch_set_tiles(cfdx_batch *q, const cfdx_tiles *t);
+
+/* ------------------------------------------------------------------------ */
+/* Bounded-block geometry (host)                                             */
+/* ------------------------------------------------------------------------ */
+/* Algebraic C-grid around a NACA 4-digit section (closed trailing edge) extruded in span: i runs
+ * from the lower wake exit along the wake to the trailing edge, around the section (lower surface,
+ * leading edge, upper surface) and out along the upper wake; j runs from the wall / wake cut to a
+ * far-field C of the given radius around the trailing edge (sinh stretching, first spacing `wall`);
+ * k is periodic over `span`. Sides: i- / i+ outflow, j- wall on the section and wake cut behind it,
+ * j+ far field. */
+typedef struct {
+    double m, p, t;        /* NACA 4-digit: maximum camber, its chordwise position, thickness (chords) */
+    double wake;           /* wake length behind the trailing edge (chords) */
+    double radius;         /* far-field radius (chords) */
+    double wall;           /* first wall-normal spacing (chords) */
+    double span;           /* span of the periodic k direction (chords) */
+    int32_t n_wake;        /* nodes on each wake line */
+    int32_t pad_;
+} cfdx_cgrid;
+int cfdx_geom_cgrid_desc(int32_t ni, int32_t nj, int32_t nk, int32_t precision, const cfdx_cgrid *g, double aoa,
+                         cfdx_block_desc *out);
+/* Coordinates of every stored node (3 per node, padded layout of cfdx_block_padded), ghosts included:
+ * ghost rows below the wall mirror the grid through it and below the wake cut are the nodes across. */
+int cfdx_geom_cgrid_coords(const cfdx_block_desc *d, const cfdx_cgrid *g, double *xyz);
+/* Coordinates of a sinusoidally mapped box of periods L over the stored nodes of any descriptor. */
+int cfdx_geom_box_coords(const cfdx_block_desc *d, const double L[3], double amp, const double ph[3], double *xyz);
+/* The 22 geometry arrays of cfdx_block_set_geometry_soa from stored-node coordinates: compact
+ * 6th-order derivatives along grid lines (Grid_Transformation.cpp:45-153; cyclic in periodic
+ * directions of period L[d], through the wake cut as one line), node metrics, half-point metrics. */
+int cfdx_geom_block_metrics(const cfdx_block_desc *d, const double *xyz, const double L[3], double *geo);
 
 /* ------------------------------------------------------------------------ */
 /* On-disk formats (host)                                                    */

@@ -1,0 +1,144 @@
+"""Bounded blocks on the GPU: side conditions, loc-code branches, the C-grid wake cut.
+
+- parity: the block path against the oracle kernels driven through the same side conditions
+  (tests/bounded.py) on a NACA C-grid and on walled boxes, 2 RK3 steps, FP64, 1e-12 relative;
+- mixed precision against FP64 on the C-grid (per-field relative L2 <= 1e-5);
+- a batch holding bounded and periodic blocks gives each block bitwise its result alone.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import bounded
+import cfdx
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if cfdx.device_count() < 1:
+        pytest.skip("no HIP device")
+    return cfdx.Ctx(0)
+
+
+def perturbed(xyz_owned, mach, aoa):
+    f = cfdx.freestream(xyz_owned.shape[0], mach, aoa)
+    x, y, z = xyz_owned[:, 0], xyz_owned[:, 1], xyz_owned[:, 2]
+    f["u"] *= 1 + 0.05 * np.sin(x) * np.cos(y)
+    f["v"] += 0.02 * np.cos(0.8 * x + 0.1 * z)
+    f["w"] += 0.01 * np.sin(0.6 * y)
+    f["p"] *= 1 + 0.02 * np.cos(0.7 * x + 0.3 * y)
+    f["rho"] *= 1 + 0.01 * np.sin(0.5 * y)
+    return f
+
+
+def cgrid_case():
+    d, g = cfdx.cgrid_desc(40, 12, 4, naca=(0.02, 0.4, 0.12), n_wake=8, aoa_deg=2.0)
+    xyz, geo = cfdx.cgrid_geometry(d, g)
+    return d, xyz, geo, 0.5, 2.0, dict(mach=0.5, reyl=1e5)
+
+
+def box_case(periodic, bc, dims, mach=0.4, reyl=800.0):
+    d = cfdx.BlockDesc(*dims, periodic, cfdx.FP64)
+    for q in range(6):
+        d.bc[q] = bc[q]
+    xyz, geo = cfdx.box_geometry(d, (1.3, 1.1, 0.9), 0.02, (0.3, 1.2, 2.1))
+    return d, xyz, geo, mach, 0.0, dict(mach=mach, reyl=reyl)
+
+
+CASES = {
+    "cgrid": cgrid_case,
+    "box-walls-j": lambda: box_case(4, (3, 2, 1, 1, 0, 0), (14, 12, 6)),
+    "box-all-bounded": lambda: box_case(0, (4, 4, 1, 2, 1, 3), (10, 9, 8)),
+}
+
+
+def run_block(ctx, d, geo, flow, mach, dt, prm, steps, precision=cfdx.FP64):
+    """A block of descriptor d: geometry, flow (side conditions applied), `steps` RK3 steps; dt
+    defaults to half the CFL-0.5 step of the initial state."""
+    dd = cfdx.BlockDesc.from_buffer_copy(d)
+    b = cfdx.Block(d.ni, d.nj, d.nk, precision=precision, ctx=ctx, desc=dd)
+    b.set_geometry_soa(geo)
+    b.configure(mach, 0.0, prm)       # the side conditions of set_flow use the block's Mach
+    b.set_flow(flow)
+    if dt is None:
+        dt = 0.5 * b.cfl_dt(0.5)
+    b.configure(mach, dt, prm)
+    if steps:
+        b.step(steps, mach, dt, prm)
+    return b, dt
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_bounded_parity(ctx, orc, name):
+    d, xyz, geo, mach, aoa, pk = CASES[name]()
+    own = cfdx.owned_index(d)
+    flow = perturbed(xyz[own], mach, aoa)
+    prm = cfdx.default_params(**pk)
+    b, dt = run_block(ctx, d, geo, flow, mach, None, prm, 2)
+    got = b.get_flow()
+    import pyoracle
+    rc = bounded.reference_case(d, geo, flow, mach)
+    want = bounded.oracle_steps(orc, rc, 2, mach, dt, pyoracle.reference_params(**pk))
+    vel = np.sqrt(sum(want[c] ** 2 for c in "uvw")).max()
+    for f in FIELDS:
+        scale = vel if f in "uvw" else np.abs(want[f]).max()
+        err = np.abs(got[f] - want[f]).max() / scale
+        print(f"{name} {f}: {err:.2e}")
+        assert err <= 1e-12, (name, f, err)
+    assert all(np.isfinite(got[f]).all() for f in FIELDS)
+
+
+def test_cgrid_mixed_vs_fp64(ctx):
+    d, xyz, geo, mach, aoa, pk = cgrid_case()
+    own = cfdx.owned_index(d)
+    flow = perturbed(xyz[own], mach, aoa)
+    prm = cfdx.default_params(**pk)
+    b64, dt = run_block(ctx, d, geo, flow, mach, None, prm, 10)
+    b32, _ = run_block(ctx, d, geo, flow, mach, dt, prm, 10, precision=cfdx.MIXED)
+    a, r = b32.get_flow(), b64.get_flow()
+    vel = np.sqrt(sum(r[c] ** 2 for c in "uvw"))
+    errs = {}
+    for f in FIELDS:
+        ref = vel if f in "uvw" else np.abs(r[f])
+        errs[f] = np.linalg.norm(a[f] - r[f]) / np.linalg.norm(ref)
+    print("cgrid mixed vs fp64 rel L2:", {f: f"{e:.2e}" for f, e in errs.items()})
+    assert all(e <= 1e-5 for e in errs.values()), errs
+
+
+def test_batch_with_bounded_blocks(ctx):
+    prm = cfdx.default_params(mach=0.5, reyl=1e5)
+    alone, blocks = [], []
+    for d, xyz, geo, mach, aoa, pk in (cgrid_case(), box_case(0, (4, 4, 1, 2, 1, 3), (10, 9, 8))):
+        own = cfdx.owned_index(d)
+        flow = perturbed(xyz[own], mach, aoa)
+        p = cfdx.default_params(**pk)
+        b, dt = run_block(ctx, d, geo, flow, mach, None, p, 3)
+        alone.append(b.get_flow())
+        b2, _ = run_block(ctx, d, geo, flow, mach, dt, p, 0)
+        blocks.append(b2)
+
+    def tgv():
+        b = cfdx.Block(12, 10, 8, ctx=ctx)
+        b.init_case(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05, phases=(0.1, 0.2, 0.3), init="tgv", mach=0.3)
+        return b
+    ref_per = tgv()
+    dtp = ref_per.cfl_dt(0.5)
+    ref_per.step(3, 0.3, dtp, prm)
+    alone.append(ref_per.get_flow())
+    per = tgv()
+    per.configure(0.3, dtp, prm)
+    blocks.append(per)
+    q = cfdx.Batch(cfdx.FP64, ctx)
+    for b in blocks:
+        q.add(b)
+    q.step(3, True)
+    for b, want in zip(blocks, alone):
+        got = b.get_flow()
+        for f in FIELDS:
+            assert np.array_equal(got[f], want[f]), f
+    q.close()
