@@ -8,8 +8,10 @@ the state resident in HBM. Prints one JSON line.
 
 Beside the headline the same JSON line carries (unless --legs drops them): the other precision
 on the same case with per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the 64-cube batched
-mixed-precision graph step ("batched_c4"), and a bounded sample of a Poisson request stream served
-by continuous batching vs plain FP64 one at a time ("serving": p50 / p99 latency, throughput).
+mixed-precision graph step ("batched_c4"), the NACA 0012 C-grid in mixed and plain FP64 with
+per-field errors ("airfoil_c3"), and a bounded sample of the Poisson stream of NACA C-grid solves
+served by continuous batching vs plain FP64 one at a time ("serving": p50 / p99 latency,
+throughput).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1] [--precision fp64|mixed]
                     [--legs compare,c4,serve] [--no-cpu]
@@ -240,19 +242,68 @@ def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int
             "speedup_vs_plain_fp64": ms_plain / ms, "state_finite": finite}
 
 
-def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202):
+def airfoil_c3(cfdx, steps: int):
+    """NACA 0012 C-grid 384x128x32 (far field 20 chords, wall spacing 1e-4 chord, span 0.25), Mach
+    0.5, AoA 2 deg, Re 1e5, MTS LES: the mixed form and the plain FP64 form from the same free
+    stream for `steps` RK3 steps; throughput of both and per-field errors of mixed against FP64."""
+    import numpy as np
+    ctx = cfdx.Ctx(0)
+    d, g = cfdx.cgrid_desc(384, 128, 32, naca=(0.0, 0.4, 0.12), wake=20.0, radius=20.0, wall=1e-4, span=0.25,
+                           aoa_deg=2.0)
+    t0 = time.perf_counter()
+    _, geo = cfdx.cgrid_geometry(d, g)
+    t_geo = time.perf_counter() - t0
+    n = 384 * 128 * 32
+    prm = cfdx.default_params(mach=0.5, reyl=1e5)
+    flow0 = cfdx.freestream(n, 0.5, 2.0)
+    res, dt = {}, None
+    for name, prec in (("fp64", cfdx.FP64), ("mixed", cfdx.MIXED)):
+        b = cfdx.Block(384, 128, 32, precision=prec, ctx=ctx, desc=cfdx.BlockDesc.from_buffer_copy(d))
+        b.set_geometry_soa(geo)
+        b.configure(0.5, 0.0, prm)
+        b.set_flow(flow0)
+        if dt is None:
+            dt = b.cfl_dt(0.5)
+        b.configure(0.5, dt, prm)
+        ctx.sync()
+        ctx.record(0)
+        b.step(steps, 0.5, dt, prm)
+        ctx.record(1)
+        ctx.sync()
+        ms = ctx.elapsed_ms(0, 1)
+        res[name] = (b.get_flow(), n * steps / (ms / 1e3), ms / steps)
+        b.close()
+    errs = field_errors(res["mixed"][0], res["fp64"][0])
+    finite = all(bool(np.isfinite(cfdx_matrix(res[k][0])).all()) for k in res)
+    return {"workload": "NACA 0012 C-grid 384x128x32, far field 20c, wall 1e-4c, Mach 0.5, AoA 2 deg, Re 1e5, MTS LES",
+            "steps": steps, "dt": dt, "geometry_s": round(t_geo, 3),
+            "fp64_value": res["fp64"][1], "mixed_value": res["mixed"][1], "unit": "cell-updates/s",
+            "fp64_ms_per_step": res["fp64"][2], "mixed_ms_per_step": res["mixed"][2],
+            "mixed_over_fp64": res["mixed"][1] / res["fp64"][1], "field_errors_mixed_vs_fp64": errs,
+            "state_finite": finite}
+
+
+def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kind: str = "airfoil"):
     """A bounded sample of a Poisson request stream served by continuous batching (mixed) and by
     plain FP64 one request at a time, on the same stream."""
+    import numpy as np
     from cfdx import serve
-    reqs = serve.periodic_stream(nreq, rate, seed=seed, nsteps=nsteps)
-    out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, grids 128x64x16..384x128x32, "
-                     "periodic TGV / vortex (curvilinear map), Mach U(0.1,0.6), Re U(500,5000)"}
+    if kind == "airfoil":
+        reqs = serve.airfoil_stream(nreq, rate, seed=seed, nsteps=nsteps)
+        out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, NACA 4-digit C-grids "
+                         "128x64x16..384x128x32, camber U(0,0.04), thickness U(0.08,0.18), AoA U(-4,10) deg, "
+                         "Mach U(0.3,0.7), Re U(1e5,1e6)"}
+    else:
+        reqs = serve.periodic_stream(nreq, rate, seed=seed, nsteps=nsteps)
+        out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, grids 128x64x16..384x128x32, "
+                         "periodic TGV / vortex (curvilinear map), Mach U(0.1,0.6), Re U(500,5000)"}
     for name, prec, batched in (("batched_mixed", cfdx.MIXED, True), ("plain_fp64", cfdx.FP64, False)):
         srv = serve.Server(precision=prec, batched=batched, chunk=10, graph=True, keep_flow=True)
         res = srv.run(reqs, clock="wall")
         rep = serve.report(reqs, res)
         rep["peak_active"] = srv.peak_active
         rep["peak_bytes"] = srv.peak_bytes
+        rep["state_finite"] = all(bool(np.isfinite(cfdx_matrix(r.flow)).all()) for r in res.values())
         srv.close()
         out[name] = rep
         print(f"bench: serving {name}: p50 {rep['p50_s']:.3f}s p99 {rep['p99_s']:.3f}s "
@@ -269,11 +320,13 @@ def main() -> int:
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--legs", default="compare,c4,serve",
-                    help="extra legs after the headline: compare (other precision, per-field errors), c4, serve")
+    ap.add_argument("--legs", default="compare,c4,c3,serve",
+                    help="extra legs after the headline: compare (other precision, per-field errors), c4, c3, serve")
+    ap.add_argument("--c3-steps", type=int, default=500)
+    ap.add_argument("--serve-kind", default="airfoil", choices=["airfoil", "periodic"])
     ap.add_argument("--c4-steps", type=int, default=5)
     ap.add_argument("--no-tune", dest="tune", action="store_false", help="default tiles instead of the autotuned ones")
-    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-steps", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
@@ -395,8 +448,12 @@ def main() -> int:
     if "c4" in legs:
         line["batched_c4"] = batched_c4(cfdx, args.c4_steps, 1)
         print(f"bench: c4 {line['batched_c4']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    if "c3" in legs:
+        line["airfoil_c3"] = airfoil_c3(cfdx, args.c3_steps)
+        print(f"bench: c3 mixed {line['airfoil_c3']['mixed_value']:.3e} fp64 {line['airfoil_c3']['fp64_value']:.3e}",
+              file=sys.stderr, flush=True)
     if "serve" in legs:
-        line["serving"] = serving(cfdx, args.serve_requests, args.serve_rate, args.serve_steps)
+        line["serving"] = serving(cfdx, args.serve_requests, args.serve_rate, args.serve_steps, kind=args.serve_kind)
 
     if not args.no_cpu:
         cb = cpu_baseline(cfdx)

@@ -301,15 +301,28 @@ def cgrid_desc(ni: int, nj: int, nk: int, precision: int = FP64, naca=(0.0, 0.4,
     return d, g
 
 
-def cgrid_geometry(desc: BlockDesc, g: CGrid) -> tuple[np.ndarray, np.ndarray]:
-    """(xyz of the stored nodes, the 22 geometry arrays) of a C-grid block."""
+def cgrid_geometry(desc: BlockDesc, g: CGrid, full: bool = False) -> tuple[np.ndarray, np.ndarray]:
+    """(xyz of the stored nodes, the 22 geometry arrays) of a C-grid block. The grid is a spanwise
+    extrusion, so every k plane has the same metrics: they are computed on a 3-plane block of the
+    same spanwise spacing and repeated (full=True computes every plane)."""
     dims, _ = padded(desc)
     N = int(np.prod(dims))
-    xyz = np.zeros((N, 3))
-    _check(_lib.cfdx_geom_cgrid_coords(C.byref(desc), C.byref(g), _ptr(xyz)))
-    L = np.array([0.0, 0.0, g.span])
-    geo = np.zeros(22 * N)
-    _check(_lib.cfdx_geom_block_metrics(C.byref(desc), _ptr(xyz), _ptr(L), _ptr(geo)))
+    if full or desc.nk <= 3:
+        xyz = np.zeros((N, 3))
+        _check(_lib.cfdx_geom_cgrid_coords(C.byref(desc), C.byref(g), _ptr(xyz)))
+        L = np.array([0.0, 0.0, g.span])
+        geo = np.zeros(22 * N)
+        _check(_lib.cfdx_geom_block_metrics(C.byref(desc), _ptr(xyz), _ptr(L), _ptr(geo)))
+        return xyz, geo
+    d3 = BlockDesc.from_buffer_copy(desc)
+    d3.nk = 3
+    g3 = CGrid.from_buffer_copy(g)
+    g3.span = 3.0 * g.span / desc.nk
+    x3, geo3 = cgrid_geometry(d3, g3, full=True)
+    plane = dims[0] * dims[1]
+    xyz = np.tile(x3[:plane], (desc.nk, 1))
+    xyz[:, 2] = np.repeat(np.arange(desc.nk) * (g.span / desc.nk), plane)
+    geo = np.tile(geo3.reshape(22, 3, plane)[:, :1, :], (1, desc.nk, 1)).reshape(-1)
     return xyz, geo
 
 

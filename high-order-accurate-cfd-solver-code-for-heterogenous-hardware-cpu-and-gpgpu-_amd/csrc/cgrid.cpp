@@ -189,39 +189,57 @@ extern "C" int cfdx_geom_cgrid_coords(const cfdx_block_desc *d, const cfdx_cgrid
         beta = 0.5 * (lo + hi);
     }
     auto fj = [&](int j) { return std::sinh(beta * j / (nj - 1)) / std::sinh(beta); };
-    // far-field point of airfoil column i: arc fraction g(u) on the far semicircle of the inner
-    // arc fraction u along the section, g(u) = c u + (1 - c)(u - sin(2 pi u) / (2 pi)) with
-    // c = perimeter / (pi R), so the far spacing equals the section's at both trailing-edge
-    // columns and joins the wake columns smoothly
-    std::vector<double> arc(naf, 0.0);
-    for (int q = 1; q < naf; ++q) {
-        double xa, ya, xb, yb, t0, t1;
-        airfoil(nw + q - 1, xa, ya, t0);
-        airfoil(nw + q, xb, yb, t1);
-        arc[q] = arc[q - 1] + std::hypot(xb - xa, yb - ya);
+    // Far field: the wake columns end on the lines y = -R / +R, uniformly in x over the wake length;
+    // the airfoil columns end on a semicircle of radius R around the trailing edge, its angles
+    // clustered at both ends so that the far-field spacing is continuous at the junction with the
+    // wake. Airfoil columns are straight lines from the section to their far-field point (odd in j
+    // through the wall). Wake columns lean from the cut toward their far-field point with the even
+    // weight h(j) = f^2 / sqrt(f^2 + c^2) (normalised to 1 at the far field), c = d0 / R: they leave
+    // the cut normal to it over one trailing-edge spacing and lean like the airfoil columns beyond,
+    // and a ghost row below the cut is exactly the row across it. Beyond the far field a wake column
+    // continues by (h - 1) D(s), D the running maximum over s of the lean xf - x0: the same line near
+    // the edge, and never converging where the cut spacing outgrows the far-field spacing.
+    const double cw = d0 / g->radius, hnorm = std::sqrt(1.0 + cw * cw);
+    auto hj = [&](int j) {
+        const double e = fj(j);
+        return e * e / std::sqrt(e * e + cw * cw) * hnorm;
+    };
+    std::vector<double> lean(nw + P.off[0] + 1);
+    for (int s = 0; s < (int)lean.size(); ++s) {
+        const double d = 1.0 + g->wake * (double)s / (double)nw - wake_x(s);
+        lean[s] = s ? std::max(lean[s - 1], d) : d;
     }
-    const double perim = arc[naf - 1], cfrac = std::min(1.0, perim / (M_PI * g->radius));
+    const double dth_end = (g->wake / nw) / g->radius;          // wanted end angle step
+    double amod = dth_end * (naf - 1) / M_PI - 1.0;            // theta(t) = pi (t + a sin(2 pi t) / (2 pi))
+    amod = std::min(0.8, std::max(-0.8, amod));
+    auto theta = [&](int i) {
+        const double t = (double)(i - nw) / (double)(naf - 1);
+        return -0.5 * M_PI - M_PI * (t + amod * std::sin(2.0 * M_PI * t) / (2.0 * M_PI));
+    };
     for (int k = -P.off[2]; k < nk + P.off[2]; ++k)
         for (int j = -P.off[1]; j < nj + P.off[1]; ++j)
             for (int i = -P.off[0]; i < ni + P.off[0]; ++i) {
-                double x0, y0, xf, yf;
-                if (i < nw) {                       // lower wake, behind the edge
-                    x0 = wake_x(nw - i); y0 = 0.0; xf = x0; yf = -g->radius;
-                } else if (i > ni - 1 - nw) {       // upper wake
-                    x0 = wake_x(i - (ni - 1 - nw)); y0 = 0.0; xf = x0; yf = g->radius;
-                } else {
-                    airfoil(i, x0, y0, tp);
-                    const double u = arc[i - nw] / perim;
-                    const double gu = cfrac * u + (1.0 - cfrac) * (u - std::sin(2.0 * M_PI * u) / (2.0 * M_PI));
-                    const double th = -0.5 * M_PI - M_PI * gu;
-                    xf = 1.0 + g->radius * std::cos(th);
-                    yf = g->radius * std::sin(th);
-                }
+                double X, Y;
                 const double f = fj(j);
-                double *X = xyz + 3 * P.at(i, j, k);
-                X[0] = x0 + f * (xf - x0);
-                X[1] = y0 + f * (yf - y0);
-                X[2] = k * (g->span / nk);
+                if (i < nw || i > ni - 1 - nw) {
+                    const bool lower = i < nw;
+                    const int s = lower ? nw - i : i - (ni - 1 - nw);   // steps behind the edge
+                    const double x0 = wake_x(s), side = lower ? -1.0 : 1.0;
+                    const double xf = 1.0 + g->wake * (double)s / (double)nw;
+                    X = j < nj ? x0 + hj(j) * (xf - x0) : xf + (hj(j) - 1.0) * lean[s];
+                    Y = f * side * g->radius;
+                } else {
+                    double x0, y0;
+                    airfoil(i, x0, y0, tp);
+                    const double th = theta(i);
+                    const double xf = 1.0 + g->radius * std::cos(th), yf = g->radius * std::sin(th);
+                    X = x0 + f * (xf - x0);
+                    Y = y0 + f * (yf - y0);
+                }
+                double *Xp = xyz + 3 * P.at(i, j, k);
+                Xp[0] = X;
+                Xp[1] = Y;
+                Xp[2] = k * (g->span / nk);
             }
     return CFDX_OK;
 }

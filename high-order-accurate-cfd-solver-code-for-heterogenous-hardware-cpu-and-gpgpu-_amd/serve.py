@@ -8,19 +8,25 @@ free memory (hipMemGetInfo minus a reserve, optionally capped by a byte budget).
 arithmetic does not depend on what else is in the batch, so its result is the same alone and
 in the mix (the tests check bitwise equality).
 
+Requests are periodic blocks set up on the device (TGV / isentropic vortex on a mapped box) or
+NACA 4-digit C-grid solves (grid = "cgrid"), whose geometry is built on the host by worker threads
+as requests arrive, overlapping the GPU work.
+
 Clocks: "wall" takes arrival times in seconds from the start of run(); "steps" measures arrival
 in RK3 steps executed by the server, which makes a schedule reproducible for tests.
 """
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import dataclasses
 import math
 import time
 
 import numpy as np
 
-from . import FP64, MIXED, Batch, Block, Ctx, default_params, euler_params
+from . import (FP64, MIXED, Batch, Block, BlockDesc, Ctx, cgrid_desc, cgrid_geometry, default_params, euler_params,
+               freestream)
 
 
 @dataclasses.dataclass
@@ -42,10 +48,20 @@ class Request:
     beta: float = 5.0
     center: tuple = (5.0, 5.0)
     cfl: float = 0.5
+    naca: tuple = (0.0, 0.4, 0.12)   # grid == "cgrid": (max camber, its position, thickness)
+    aoa_deg: float = 0.0             # grid == "cgrid": far-field flow angle
 
     @property
     def nodes(self) -> int:
         return self.ni * self.nj * self.nk
+
+    def desc(self, precision: int) -> BlockDesc | None:
+        if self.grid != "cgrid":
+            return None
+        return cgrid_desc(self.ni, self.nj, self.nk, precision, naca=self.naca, aoa_deg=self.aoa_deg)[0]
+
+    def bytes(self, precision: int) -> int:
+        return Block.bytes_for(self.ni, self.nj, self.nk, precision, desc=self.desc(precision))
 
     def params(self):
         if not self.viscous:
@@ -67,13 +83,31 @@ class Result:
         return self.finished - self.arrival
 
 
-def make_block(req: Request, precision: int, ctx: Ctx) -> tuple[Block, float]:
-    """Creates, initialises (on the device) and configures the block of one request; returns it
-    with its CFL time step (computed once from the initial state)."""
-    b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx)
-    amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
-    b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
-                beta=req.beta, center=req.center)
+def prepare(req: Request) -> dict | None:
+    """Host-side set-up of a request (C-grid geometry and the free stream); None when the block is
+    set up on the device. Runs in worker threads."""
+    if req.grid != "cgrid":
+        return None
+    d, g = cgrid_desc(req.ni, req.nj, req.nk, FP64, naca=req.naca, aoa_deg=req.aoa_deg)
+    _, geo = cgrid_geometry(d, g)
+    return {"desc": d, "geo": geo, "flow": freestream(req.nodes, req.mach, req.aoa_deg)}
+
+
+def make_block(req: Request, precision: int, ctx: Ctx, prep: dict | None = None) -> tuple[Block, float]:
+    """Creates, initialises and configures the block of one request; returns it with its CFL time
+    step (computed once from the initial state)."""
+    if req.grid == "cgrid":
+        prep = prep or prepare(req)
+        d = BlockDesc.from_buffer_copy(prep["desc"])
+        b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx, desc=d)
+        b.set_geometry_soa(prep["geo"])
+        b.configure(req.mach, 0.0, req.params())      # the side conditions use the block's Mach
+        b.set_flow(prep["flow"])
+    else:
+        b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx)
+        amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
+        b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
+                    beta=req.beta, center=req.center)
     dt = b.cfl_dt(req.cfl)
     b.configure(req.mach, dt, req.params())
     return b, dt
@@ -91,7 +125,7 @@ def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) ->
 class Server:
     def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
                  budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
-                 grid_cap: int = 0, batched: bool = True, keep_flow: bool = True):
+                 grid_cap: int = 0, batched: bool = True, keep_flow: bool = True, prep_workers: int = 8):
         self.ctx = ctx or Ctx()
         self.precision = precision
         self.reserve = reserve_bytes
@@ -106,9 +140,10 @@ class Server:
         self.peak_active = 0
         self.peak_bytes = 0
         self.chunks = 0
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, prep_workers))
 
     def _fits(self, req: Request, used: int) -> bool:
-        need = Block.bytes_for(req.ni, req.nj, req.nk, self.precision)
+        need = req.bytes(self.precision)
         if self.budget is not None and used + need > self.budget:
             return False
         free, _ = self.ctx.mem_info()
@@ -128,16 +163,26 @@ class Server:
         def now() -> float:
             return float(vclock) if clock == "steps" else time.perf_counter() - t0
 
+        preps: dict[int, concurrent.futures.Future] = {}
         while pending or queue or active:
             t = now()
             while pending and pending[0].arrival <= t:
-                queue.append(pending.popleft())
-            # admission: FIFO while memory and slots allow
+                req = pending.popleft()
+                queue.append(req)
+                if req.grid == "cgrid":
+                    preps[req.rid] = self.pool.submit(prepare, req)
+            # admission: FIFO while memory and slots allow, once the host set-up is ready (it is
+            # waited for when nothing else could run)
             while queue and len(active) < self.max_active and self._fits(queue[0], used):
-                req = queue.popleft()
-                b, dt = make_block(req, self.precision, self.ctx)
+                req = queue[0]
+                fut = preps.get(req.rid)
+                if fut is not None and not fut.done() and active:
+                    break
+                queue.popleft()
+                prep = preps.pop(req.rid).result() if fut is not None else None
+                b, dt = make_block(req, self.precision, self.ctx, prep)
                 self.batch.add(b)
-                nb = Block.bytes_for(req.ni, req.nj, req.nk, self.precision)
+                nb = req.bytes(self.precision)
                 used += nb
                 active[req.rid] = dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb)
             if queue and not active:
@@ -178,6 +223,7 @@ class Server:
 
     def close(self) -> None:
         self.batch.close()
+        self.pool.shutdown(wait=True)
 
 
 def report(requests: list[Request], results: dict[int, Result]) -> dict:
@@ -211,4 +257,22 @@ def periodic_stream(n: int, rate: float, seed: int = 20261202, sizes=((128, 64, 
         else:
             out.append(Request(rid, ni, nj, nk, nsteps, t, grid="sinusoid", length=(10.0,) * 3, phases=phases,
                                init="vortex", mach=mach, reyl=reyl, center=(5.0, 5.0)))
+    return out
+
+
+C5_LADDER = ((128, 64, 16), (192, 96, 24), (256, 128, 32), (384, 128, 32))
+
+
+def airfoil_stream(n: int, rate: float = 20.0, seed: int = 20261202, sizes=C5_LADDER, nsteps: int = 300) -> list[Request]:
+    """Poisson arrivals of NACA 4-digit C-grid solves: camber U(0, 0.04) at 0.4 chord, thickness
+    U(0.08, 0.18), AoA U(-4, 10) deg, Mach U(0.3, 0.7), Re U(1e5, 1e6), grid sizes from the ladder."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        ni, nj, nk = sizes[rng.integers(len(sizes))]
+        naca = (float(rng.uniform(0.0, 0.04)), 0.4, float(rng.uniform(0.08, 0.18)))
+        out.append(Request(rid, ni, nj, nk, nsteps, t, grid="cgrid", naca=naca, aoa_deg=float(rng.uniform(-4.0, 10.0)),
+                           mach=float(rng.uniform(0.3, 0.7)), reyl=float(rng.uniform(1e5, 1e6))))
     return out

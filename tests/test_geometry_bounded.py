@@ -3,6 +3,7 @@ C-grid and its wake cut."""
 import math
 
 import numpy as np
+import pytest
 
 import bounded
 import cfdx
@@ -88,9 +89,27 @@ def test_cgrid_symmetric_and_cut():
     assert c[L.g(ni // 2, -1, 0)] == 100
 
 
-def test_cgrid_ladder_valid():
-    for ni, nj, nk in ((128, 64, 16), (192, 96, 24), (256, 128, 32)):
-        d, g = cfdx.cgrid_desc(ni, nj, nk, naca=(0.02, 0.4, 0.12))
+@pytest.mark.parametrize("dims", [(128, 64, 16), (192, 96, 24), (256, 128, 32), (384, 128, 32), (48, 16, 8)])
+def test_cgrid_ladder_valid(dims):
+    """Every stored node (ghosts included) has a positive Jacobian over the range of sections the
+    airfoil request stream draws (thickness 8-18 %, camber 0-4 %) and the serving grid sizes."""
+    ni, nj, nk = dims
+    for naca in ((0.0, 0.4, 0.08), (0.04, 0.4, 0.18), (0.02, 0.4, 0.12)):
+        d, g = cfdx.cgrid_desc(ni, nj, nk, naca=naca, n_wake=12 if ni == 48 else None)
         xyz, geo = cfdx.cgrid_geometry(d, g)
         N = xyz.shape[0]
-        assert np.isfinite(geo).all() and (geo[:N] > 0).all()
+        assert np.isfinite(geo).all() and (geo[:N] > 0).all(), (dims, naca)
+
+
+def test_cgrid_plane_replication_matches_full():
+    d, g = cfdx.cgrid_desc(48, 16, 8, naca=(0.03, 0.4, 0.15), n_wake=12, aoa_deg=3.0)
+    x1, g1 = cfdx.cgrid_geometry(d, g)
+    x2, g2 = cfdx.cgrid_geometry(d, g, full=True)
+    assert np.allclose(x1, x2, rtol=0, atol=1e-15)
+    N = x1.shape[0]
+    a, b = g1.reshape(22, N), g2.reshape(22, N)
+    # det, the node metrics, the face rows and the face dets, each against its group's scale (the
+    # xi_x, xi_y components vanish up to round-off of the spanwise closures)
+    for lo, hi in ((0, 1), (1, 10), (10, 19), (19, 22)):
+        scale = np.abs(b[lo:hi]).max()
+        assert np.abs(a[lo:hi] - b[lo:hi]).max() <= 1e-12 * scale, (lo, hi)

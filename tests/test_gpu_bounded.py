@@ -37,7 +37,7 @@ def perturbed(xyz_owned, mach, aoa):
 
 
 def cgrid_case():
-    d, g = cfdx.cgrid_desc(40, 12, 4, naca=(0.02, 0.4, 0.12), n_wake=8, aoa_deg=2.0)
+    d, g = cfdx.cgrid_desc(48, 16, 4, naca=(0.02, 0.4, 0.12), n_wake=12, aoa_deg=2.0)
     xyz, geo = cfdx.cgrid_geometry(d, g)
     return d, xyz, geo, 0.5, 2.0, dict(mach=0.5, reyl=1e5)
 

@@ -106,3 +106,28 @@ def test_serving_wall_clock(ctx):
     print("serving (wall clock):", rep)
     assert rep["requests"] == len(reqs) and rep["cell_updates_per_s"] > 0
     assert all(res[r.rid].finished >= res[r.rid].admitted >= r.arrival - 1e-9 for r in reqs)
+
+
+def test_serving_airfoils_alone_vs_mix(ctx):
+    """NACA C-grid requests (geometry built on the host by the worker threads as they arrive) and
+    periodic requests in one batch: each result is bitwise the request alone."""
+    reqs = [
+        serve.Request(0, 48, 16, 4, nsteps=8, arrival=0, grid="cgrid", naca=(0.02, 0.4, 0.12), aoa_deg=2.0,
+                      mach=0.5, reyl=2e5),
+        serve.Request(1, 16, 12, 10, nsteps=6, arrival=2, grid="sinusoid", length=(2 * math.pi,) * 3,
+                      phases=(0.2, 0.7, 1.3), init="tgv", mach=0.2, reyl=800.0),
+        serve.Request(2, 56, 20, 6, nsteps=10, arrival=3, grid="cgrid", naca=(0.0, 0.4, 0.15), aoa_deg=-3.0,
+                      mach=0.4, reyl=5e5),
+        serve.Request(3, 48, 16, 8, nsteps=5, arrival=7, grid="cgrid", naca=(0.04, 0.4, 0.09), aoa_deg=8.0,
+                      mach=0.65, reyl=1e6),
+    ]
+    srv = serve.Server(precision=cfdx.MIXED, ctx=ctx, reserve_bytes=0, chunk=3, prep_workers=2)
+    res = srv.run(reqs, clock="steps")
+    srv.close()
+    assert sorted(res) == [r.rid for r in reqs] and srv.peak_active >= 2
+    for r in reqs:
+        alone = serve.run_alone(r, cfdx.MIXED, ctx)
+        assert res[r.rid].dt == alone.dt
+        for f in FIELDS:
+            assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
+            assert np.isfinite(res[r.rid].flow[f]).all()
