@@ -176,13 +176,16 @@ void gather_owned(const cfdx_block *b, const double *stored, double *owned)
 }
 
 // side-condition extent of one block (largest side face of a bounded direction) and wake cut
-void bc_extent(const BlockDev &B, uint32_t &face, bool &cut)
+void bc_extent(const BlockDev &B, LaunchList &L)
 {
     if (!B.code) return;
-    if (B.off[0]) face = std::max(face, (uint32_t)B.nj * B.nk);
-    if (B.off[1]) face = std::max(face, (uint32_t)B.ni * B.nk);
-    if (B.off[2]) face = std::max(face, (uint32_t)B.ni * B.nj);
-    if (B.off[1] && B.bc[2] == CFDX_BC_WALL_CUT) cut = true;
+    const uint32_t f[3] = {(uint32_t)B.nj * B.nk, (uint32_t)B.ni * B.nk, (uint32_t)B.ni * B.nj};
+    for (int d = 0; d < 3; ++d)
+        if (B.off[d]) {
+            L.bc_face_d[d] = std::max(L.bc_face_d[d], f[d]);
+            L.bc_face = std::max(L.bc_face, f[d]);
+        }
+    if (B.off[1] && B.bc[2] == CFDX_BC_WALL_CUT) L.any_cut = true;
 }
 
 // single-block launch list (tile offsets 0 / total)
@@ -209,7 +212,7 @@ LaunchList single_list(cfdx_block *b)
     L.blks = b->dB;
     L.nblk = 1;
     L.tiles = b->tiles;
-    bc_extent(b->B, L.bc_face, L.any_cut);
+    bc_extent(b->B, L);
     for (int kind = 0; kind < 4; ++kind) {
         L.prefix[kind] = b->dprefix + 2 * kind;
         L.total[kind] = cfdx::blk::tiles_of(b->B, kind, b->tiles);
@@ -292,8 +295,9 @@ int upload_batch(cfdx_batch *q)
     q->L.grid_cap = q->grid_cap;
     q->L.tiles = q->tiles;
     q->L.bc_face = 0;
+    for (int d = 0; d < 3; ++d) q->L.bc_face_d[d] = 0;
     q->L.any_cut = false;
-    for (size_t i = 0; i < nb; ++i) bc_extent(h[i], q->L.bc_face, q->L.any_cut);
+    for (size_t i = 0; i < nb; ++i) bc_extent(h[i], q->L);
     q->seen.resize(nb);
     for (size_t i = 0; i < nb; ++i) q->seen[i] = q->blocks[i]->version;
     q->dirty = false;

@@ -67,6 +67,7 @@ struct LaunchList {
     uint32_t grid_cap = 0;            // 0: one workgroup per tile; else persistent grid of this size
     TileCfg tiles;
     uint32_t bc_face = 0;             // largest side face (owned nodes) of the bounded blocks; 0 if none
+    uint32_t bc_face_d[3] = {};       // the same per direction (0: no block bounded in that direction)
     bool any_cut = false;             // some block has a wake cut
 };
 

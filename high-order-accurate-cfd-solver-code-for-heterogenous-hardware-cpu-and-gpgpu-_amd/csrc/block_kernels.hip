@@ -70,6 +70,18 @@ __device__ __host__ __forceinline__ int faces_along(const BlockDev &B, int d)
     return B.off[d] ? n + 1 : n;
 }
 
+// The loc-selected forms of a bounded block reduce to the central ones unless the node is a side
+// node (code 1) or a side node or ghost (code 100) lies within 2 nodes of it along d; elsewhere
+// the kernels take the central forms directly, so a wave away from the sides keeps the short path.
+__device__ __forceinline__ bool near_side(const BlockDev &B, int i, int j, int k, int d)
+{
+    const int c[3] = {i, j, k}, n[3] = {B.ni, B.nj, B.nk};
+    bool s = false;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s |= B.off[q] && (c[q] == 0 || c[q] == n[q] - 1);
+    return s || (B.off[d] && (c[d] < 3 || c[d] > n[d] - 4));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each
 // with its own L2. Remap the linear workgroup id so XCD x takes one contiguous range of tiles
 // (a slab of k planes of one block). Bijective for any count.
@@ -120,7 +132,7 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
             const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
             const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
             double um1[5], up1[5];
-            if (BND && B.code) {
+            if (BND && B.code && near_side(B, i, j, k, d)) {
                 // bounded block: the loc-selected one-sided forms of Viscous.cl:285-435 near the sides
                 const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
                 const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
@@ -303,7 +315,7 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
                 for (int d = 0; d < 3; ++d) {
                     const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
                     const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
-                    if (BND && B.code) {
+                    if (BND && B.code && near_side(B, i, j, k, d)) {
                         // bounded block: the loc-selected forms of Weno.cl:2732-2856
                         const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
                         const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
@@ -462,46 +474,67 @@ __device__ __forceinline__ void get_prim(const BlockDev &B, uint32_t g, double (
 
 // primitives of a boundary or ghost node; its RK base (mode 2) or stage vector (modes 0, 1)
 // follows from them, as U_Q rebuilds U from FLOW (ghost stage vectors are not defined by the
-// reference at stages 1 and 2; here they are det * Q of the side condition's state)
+// reference at stages 1 and 2; here they are det * Q of the side condition's state). The
+// primitives are rounded to S first, as if read back from the stored arrays.
 template <typename S>
 __device__ __forceinline__ void put_node(const BlockDev &B, uint32_t g, const double (&w)[5], int mode, bool ghost)
 {
+    double r[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) r[c] = (double)S(w[c]);
+    const bool u = mode == 2 || ghost;
+    const double det = u ? ld(B.det, g) : 0.0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) st((S *)B.P[c], g, S(w[c]));
-    if (mode != 2 && !ghost) return;
-    double r[5];
-    get_prim<S>(B, g, r);
+    if (!u) return;
     double q[5];
     prim_to_q(r[0], r[1], r[2], r[3], r[4], q);
-    const double det = ld(B.det, g);
 #pragma unroll
     for (int c = 0; c < 5; ++c) st(mode == 2 ? B.U0[c] : B.Us[c], g, det * q[c]);
 }
 
 // the node across a wake cut into a ghost or duplicate node: primitives copied, the stage vector
-// and RK base carry the same Q with the destination's own det (U = det Q)
-__device__ __forceinline__ void copy_node(const BlockDev &B, uint32_t dst, uint32_t src, int ps)
+// and RK base carry the same Q with the destination's own det (U = det Q). Every value is loaded
+// before the first store.
+template <typename S>
+__device__ __forceinline__ void copy_node(const BlockDev &B, uint32_t dst, uint32_t src)
 {
     const double ratio = ld(B.det, dst) / ld(B.det, src);
+    S p[5];
+    double us[5], u0[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        if (ps == 8) st((double *)B.P[c], dst, ld((const double *)B.P[c], src));
-        else st((float *)B.P[c], dst, ld((const float *)B.P[c], src));
-        st(B.Us[c], dst, ld(B.Us[c], src) * ratio);
-        st(B.U0[c], dst, ld(B.U0[c], src) * ratio);
+        p[c] = ld((const S *)B.P[c], src);
+        us[c] = ld(B.Us[c], src);
+        u0[c] = ld(B.U0[c], src);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        st((S *)B.P[c], dst, p[c]);
+        st(B.Us[c], dst, us[c] * ratio);
+        st(B.U0[c], dst, u0[c] * ratio);
     }
 }
 
+// The two sides of direction d (blockIdx.z = 0: low side, 1: high side), one thread per node of
+// a side's face and layer m (0: the boundary node, 1..3: the ghost layers). The nodes a side
+// writes are its boundary nodes and ghost layers; it reads the first owned layers in from its
+// side (a bounded direction has >= 7 nodes), so the two sides of a direction are independent and
+// the directions run one after the other (i, then j, then k: a node on two sides ends with the
+// later direction's value).
 template <typename S>
-__global__ void __launch_bounds__(256) k_bc(const BlockDev *__restrict__ blks, int side, int mode)
+__global__ void __launch_bounds__(256) k_bc(const BlockDev *__restrict__ blks, int d, int mode)
 {
     const BlockDev &B = blks[blockIdx.y];
-    const int d = side >> 1, hi = side & 1;
     if (!B.off[d]) return;
+    const int hi = (int)blockIdx.z, side = 2 * d + hi;
     const int na = d == 0 ? B.nj : B.ni, nb = d == 2 ? B.nj : B.nk;
+    const uint32_t nf = (uint32_t)na * (uint32_t)nb;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint32_t)na * (uint32_t)nb) return;
-    const int a = (int)(t % (uint32_t)na), bq = (int)(t / (uint32_t)na);
+    if (t >= 4 * nf) return;
+    const int m = (int)(t / nf);
+    const uint32_t r = t - (uint32_t)m * nf;
+    const int a = (int)(r % (uint32_t)na), bq = (int)(r / (uint32_t)na);
     const int n = d == 0 ? B.ni : d == 1 ? B.nj : B.nk;
     auto at = [&](int c) -> uint32_t {
         if (d == 0) return node_index(B, c, a, bq);
@@ -517,68 +550,67 @@ __global__ void __launch_bounds__(256) k_bc(const BlockDev *__restrict__ blks, i
             // wake cut (j- side, a = i): node (i, -m) is node (ni-1-i, m); row 0 of the lower half
             // copies the upper half
             const int im = B.ni - 1 - a;
-            for (int m = 1; m <= 3; ++m) copy_node(B, node_index(B, a, -m, bq), node_index(B, im, m, bq), (int)sizeof(S));
-            if (a < im) copy_node(B, node_index(B, a, 0, bq), node_index(B, im, 0, bq), (int)sizeof(S));
+            if (m > 0) copy_node<S>(B, node_index(B, a, -m, bq), node_index(B, im, m, bq));
+            else if (a < im) copy_node<S>(B, node_index(B, a, 0, bq), node_index(B, im, 0, bq));
             return;
         }
     }
     const double M2 = 1.4 * B.mach * B.mach;
-    double w1[5], w0[5];
-    get_prim<S>(B, at(c0 + in), w1);
-    if (kind == CFDX_BC_WALL) {
-        const double t1 = M2 * (w1[4] / w1[0]);
-        w0[0] = M2 * (w1[4] / t1); w0[1] = 0.0; w0[2] = 0.0; w0[3] = 0.0; w0[4] = w1[4];
-        put_node<S>(B, at(c0), w0, mode, false);
-        for (int m = 1; m <= 3; ++m) {
-            double wm[5];
-            get_prim<S>(B, at(c0 + in * m), wm);
-            const double wg[5] = {wm[0], -wm[1], -wm[2], -wm[3], wm[4]};
-            put_node<S>(B, at(c0 - in * m), wg, mode, true);
+    double w[5];
+    if (kind == CFDX_BC_WALL || kind == CFDX_BC_EXTRAP) {
+        get_prim<S>(B, at(c0 + in * (m ? m : 1)), w);
+        if (m == 0) {
+            if (kind == CFDX_BC_WALL) {
+                const double t1 = M2 * (w[4] / w[0]);
+                w[0] = M2 * (w[4] / t1); w[1] = 0.0; w[2] = 0.0; w[3] = 0.0;
+            }
+        } else if (kind == CFDX_BC_WALL) {
+            w[1] = -w[1]; w[2] = -w[2]; w[3] = -w[3];
         }
-        return;
-    }
-    if (kind == CFDX_BC_EXTRAP) {
-        put_node<S>(B, at(c0), w1, mode, false);
-        for (int m = 1; m <= 3; ++m) {
-            double wm[5];
-            get_prim<S>(B, at(c0 + in * m), wm);
-            put_node<S>(B, at(c0 - in * m), wm, mode, true);
-        }
-        return;
-    }
-    if (kind == CFDX_BC_FARFIELD) {
+    } else if (kind == CFDX_BC_FARFIELD) {
         const double p = 1.0 / M2;
-        w0[0] = M2 * (p / 1.0); w0[1] = B.cos_aoa; w0[2] = B.sin_aoa; w0[3] = 0.0; w0[4] = p;
+        w[0] = M2 * (p / 1.0); w[1] = B.cos_aoa; w[2] = B.sin_aoa; w[3] = 0.0; w[4] = p;
     } else {   // CFDX_BC_OUTFLOW
-        const double t1 = M2 * (w1[4] / w1[0]);
-        w0[0] = M2 * (w1[4] / t1); w0[1] = w1[1]; w0[2] = w1[2]; w0[3] = w1[3]; w0[4] = w1[4];
+        get_prim<S>(B, at(c0 + in), w);
+        const double t1 = M2 * (w[4] / w[0]);
+        w[0] = M2 * (w[4] / t1);
     }
-    put_node<S>(B, at(c0), w0, mode, false);
-    for (int m = 1; m <= 3; ++m) put_node<S>(B, at(c0 - in * m), w0, mode, true);
+    put_node<S>(B, at(c0 - in * m), w, mode, m > 0);
 }
 
 // viscous node fluxes of the ghost rows across a wake cut: the mirror node's, with the signs of
-// the i and j directions reversed (zeta and eta run the other way on the other side of the cut)
+// the i and j directions reversed (zeta and eta run the other way on the other side of the cut);
+// one thread per (i, layer m, k)
 template <typename S>
 __global__ void __launch_bounds__(256) k_cut_viscous(const BlockDev *__restrict__ blks)
 {
     const BlockDev &B = blks[blockIdx.y];
     if (!B.off[1] || B.bc[2] != CFDX_BC_WALL_CUT || !B.viscous) return;
+    const uint32_t nf = (uint32_t)B.ni * (uint32_t)B.nk;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint32_t)B.ni * (uint32_t)B.nk) return;
-    const int i = (int)(t % (uint32_t)B.ni), k = (int)(t / (uint32_t)B.ni);
+    if (t >= 3 * nf) return;
+    const int m = 1 + (int)(t / nf);
+    const uint32_t r = t % nf;
+    const int i = (int)(r % (uint32_t)B.ni), k = (int)(r / (uint32_t)B.ni);
     if (i >= B.cut_i0 && i <= B.cut_i1) return;
-    const int im = B.ni - 1 - i;
-    for (int m = 1; m <= 3; ++m) {
-        const uint32_t dst = node_index(B, i, -m, k), src = node_index(B, im, m, k);
+    const uint32_t dst = node_index(B, i, -m, k), src = node_index(B, B.ni - 1 - i, m, k);
+    S v[3][4];
 #pragma unroll
-        for (int dd = 0; dd < 3; ++dd)
+    for (int dd = 0; dd < 3; ++dd)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const S v = ld((const S *)B.Fv[dd][c], src);
-                st((S *)B.Fv[dd][c], dst, dd == 2 ? v : -v);
-            }
-    }
+        for (int c = 0; c < 4; ++c) v[dd][c] = ld((const S *)B.Fv[dd][c], src);
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) st((S *)B.Fv[dd][c], dst, dd == 2 ? v[dd][c] : -v[dd][c]);
+}
+
+template <typename S>
+void launch_bc(const LaunchList &L, int mode, hipStream_t st)
+{
+    for (int d = 0; d < 3; ++d)
+        if (L.bc_face_d[d])
+            hipLaunchKernelGGL(k_bc<S>, dim3((4 * L.bc_face_d[d] + 255) / 256, L.nblk, 2), dim3(256), 0, st, L.blks, d, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -641,7 +673,7 @@ void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, 
     rec(0);
     if (any_visc) node_kernel<S>(L, grid(L.total[0]), st, true, mode);
     if (any_visc && L.any_cut)
-        hipLaunchKernelGGL(k_cut_viscous<S>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks);
+        hipLaunchKernelGGL(k_cut_viscous<S>, dim3((3 * L.bc_face_d[1] + 255) / 256, L.nblk), dim3(256), 0, st, L.blks);
     rec(1);
     switch (L.tiles.face[0]) {
     case 1: launch_face_i<S, 1>(L, grid(L.total[1]), st); break;
@@ -654,9 +686,7 @@ void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, 
     face_jk<S, 2>(L, grid(L.total[3]), st);
     rec(4);
     node_kernel<S>(L, grid(L.total[0]), st, false, mode);
-    if (L.bc_face && mode <= 2)
-        for (int side = 0; side < 6; ++side)
-            hipLaunchKernelGGL(k_bc<S>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks, side, mode);
+    if (L.bc_face && mode <= 2) launch_bc<S>(L, mode, st);
     rec(5);
 }
 
@@ -676,12 +706,8 @@ void base(const BlockDev &B, int precision, hipStream_t st, const double *stage)
 void boundary(const LaunchList &L, int precision, int mode, hipStream_t st)
 {
     if (!L.bc_face) return;
-    for (int side = 0; side < 6; ++side) {
-        if (precision == CFDX_MIXED)
-            hipLaunchKernelGGL(k_bc<float>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks, side, mode);
-        else
-            hipLaunchKernelGGL(k_bc<double>, dim3((L.bc_face + 255) / 256, L.nblk), dim3(256), 0, st, L.blks, side, mode);
-    }
+    if (precision == CFDX_MIXED) launch_bc<float>(L, mode, st);
+    else launch_bc<double>(L, mode, st);
 }
 
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
