@@ -68,11 +68,12 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def profile_counts(workload_name: str, precision: str):
+def profile_counts(cfdx, workload_name: str, precision: str, tiles=None):
     """Per-launch HBM bytes and VALU counts of the face kernels, from the committed rocprofv3
     summary of this workload (profiles/block_kernels_<workload>_<precision>.md). The summary
-    holds one machine-readable row per kernel:
-    `| kernel | launches | avg_us | hbm_bytes | valu_instr | fp64_flop | fp32_flop |`."""
+    holds one machine-readable row per kernel and tile variant,
+    `| kernel<S,DIR,TA,TB> | launches | avg_us | hbm_bytes | valu_instr | fp64_flop | fp32_flop |`;
+    the rows of the variants this run uses (tiles = face i, j, k, node variant indices) are read."""
     path = os.path.join(ROOT, "profiles", f"block_kernels_{workload_name}_{precision}.md")
     if not os.path.exists(path):
         return None
@@ -86,11 +87,24 @@ def profile_counts(workload_name: str, precision: str):
                                       fp64_flop=float(cells[5]), fp32_flop=float(cells[6]))
             except ValueError:
                 continue
-    face = [v for k, v in rows.items() if k.startswith("k_face")]
-    if len(face) != 3:
-        return None
+    t = tuple(tiles) if tiles else (0, 0, 0, 0)
+    S = "double" if precision == "fp64" else "float"
+    names = []
+    for d in range(3):
+        ta, tb = cfdx.tile_variants(0 if d == 0 else 1)[t[d]]
+        names.append(f"k_face<{S},{d},{ta},{tb}>")
+    face, used = [], []
+    for d, k in enumerate(names):
+        if k not in rows:   # the profiled run tuned another variant of this class: take its counted row
+            k = next((r for r in rows if r.startswith(f"k_face<{S},{d},")), None)
+            if k is None:
+                return None
+        face.append(rows[k])
+        used.append(k)
     mean = {k: sum(f[k] for f in face) / 3 for k in ("avg_us", "hbm_bytes", "valu_instr", "fp64_flop", "fp32_flop")}
     mean["source"] = os.path.relpath(path, ROOT)
+    mean["rows"] = used
+    mean["same_variants"] = used == names
     return mean
 
 
@@ -321,7 +335,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--legs", default="compare,c4,c3,serve",
-                    help="extra legs after the headline: compare (other precision, per-field errors), c4, c3, serve")
+                    help="extra legs after the headline: compare (other precision, per-field errors), c4, c3, serve "
+                         "(none: the headline only)")
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--serve-kind", default="airfoil", choices=["airfoil", "periodic"])
     ap.add_argument("--c4-steps", type=int, default=5)
@@ -380,7 +395,7 @@ def main() -> int:
     copy_ms = ctx.elapsed_ms(2, 3)
     a.free(); b.free()
 
-    prof = profile_counts(args.workload, args.precision)
+    prof = profile_counts(cfdx, args.workload, args.precision, tiles["tiles"] if tiles else None)
     traffic = prof["hbm_bytes"] if prof else None
     valu = None
     if prof:
@@ -391,7 +406,8 @@ def main() -> int:
                 "issue_frac": prof["valu_instr"] / sec / VALU_PEAK_WAVE_INSTR_S,
                 "fp64_tflops": prof["fp64_flop"] / sec / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                 "fp32_tflops": prof["fp32_flop"] / sec / 1e12,
-                "counts_from": prof["source"], "profiled_kernel_us": prof["avg_us"]}
+                "counts_from": prof["source"], "profile_rows": prof["rows"], "same_variants": prof["same_variants"],
+                "profiled_kernel_us": prof["avg_us"]}
 
     line = {
         "metric": "cell-updates/s",
