@@ -13,8 +13,10 @@ per-field errors ("airfoil_c3"), and a bounded sample of the Poisson stream of N
 served by continuous batching vs plain FP64 one at a time ("serving": p50 / p99 latency,
 throughput).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1] [--precision fp64|mixed]
-                    [--legs compare,c4,serve] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1|c3] [--precision fp64|mixed]
+                    [--legs compare,c4,c3,serve|none] [--no-cpu] [--no-tune]
+
+--workload c3 makes the NACA 0012 C-grid block (384x128x32, free stream start) the timed block.
 """
 from __future__ import annotations
 
@@ -61,6 +63,13 @@ def workload(name: str):
         case = dict(ni=n, nj=n, nk=n, kw=dict(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * h,
                                                phases=phases, init="tgv", mach=0.1))
         return spec, case, dict(mach=0.1, reyl=1600.0), True
+    if name == "c3":
+        spec = dict(workload="naca0012_cgrid_384x128x32_les", grid=[384, 128, 32], mach=0.5, aoa_deg=2.0, re=1e5,
+                    sgs="mixed_time_scale(C=0.03,C_T=10)", cfl=0.5,
+                    geometry="C-grid, far field 20c, wake 20c, wall 1e-4c, span 0.25c periodic")
+        case = dict(ni=384, nj=128, nk=32, cgrid=dict(naca=(0.0, 0.4, 0.12), wake=20.0, radius=20.0, wall=1e-4,
+                                                       span=0.25, aoa_deg=2.0), kw=dict(mach=0.5))
+        return spec, case, dict(mach=0.5, reyl=1e5), True
     if name == "c1":
         spec = dict(workload="isentropic_vortex_euler_64^3", grid=[64, 64, 64], mach=0.5, beta=5.0, cfl=0.5)
         case = dict(ni=64, nj=64, nk=64, kw=dict(grid="box", length=(10.0,) * 3, init="vortex", mach=0.5))
@@ -141,13 +150,37 @@ def cpu_baseline(cfdx, max_seconds: float = 15.0):
             "seconds": round(t3 - t2, 2)}
 
 
+def make_block(cfdx, cdef, precision, prm, ctx=None, host_case=None):
+    """The timed block of a workload with its CFL-0.5 step: a periodic case (uploaded from the host
+    case when given, else set up on the device) or the C-grid of `cdef["cgrid"]` from the free
+    stream."""
+    mach = cdef["kw"]["mach"]
+    if "cgrid" in cdef:
+        cg = cdef["cgrid"]
+        d, g = cfdx.cgrid_desc(cdef["ni"], cdef["nj"], cdef["nk"], precision, naca=cg["naca"], wake=cg["wake"],
+                               radius=cg["radius"], wall=cg["wall"], span=cg["span"], aoa_deg=cg["aoa_deg"])
+        if "geo" not in cg:
+            cg["geo"] = cfdx.cgrid_geometry(d, g)[1]
+        b = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision, ctx=ctx, desc=d)
+        b.set_geometry_soa(cg["geo"])
+        b.configure(mach, 0.0, prm)
+        b.set_flow(cfdx.freestream(cdef["ni"] * cdef["nj"] * cdef["nk"], mach, cg["aoa_deg"]))
+    else:
+        b = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision, ctx=ctx)
+        if host_case is not None:
+            b.set_case(host_case)
+        else:
+            b.init_case(**cdef["kw"])
+    dt = b.cfl_dt(0.5)
+    b.configure(mach, dt, prm)
+    return b, dt
+
+
 def tuned_tiles(cfdx, cdef, precision, mach, prm):
-    """Tile variants for this grid and precision, tuned on a scratch block of the same case set up
-    on the device (the tuner runs real steps), or taken from the tile cache."""
+    """Tile variants for this grid and precision, tuned on a scratch block of the same case (the
+    tuner runs real steps), or taken from the tile cache."""
     from cfdx import tune
-    scratch = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision)
-    scratch.init_case(**cdef["kw"])
-    dt = scratch.cfl_dt(0.5)
+    scratch, dt = make_block(cfdx, cdef, precision, prm)
     res = tune.tune_block(scratch, mach, dt, prm, steps=2, cache=tune.TileCache())
     scratch.close()
     return res
@@ -270,12 +303,17 @@ def airfoil_c3(cfdx, steps: int):
     n = 384 * 128 * 32
     prm = cfdx.default_params(mach=0.5, reyl=1e5)
     flow0 = cfdx.freestream(n, 0.5, 2.0)
+    # FP64 from the free stream with u perturbed by 1e-7 relative (seeded): how much the flow itself
+    # amplifies a perturbation of the size of one FP32 rounding over the same steps
+    rng = np.random.default_rng(20261202)
+    flow_p = flow0.copy()
+    flow_p["u"] *= 1 + 1e-7 * rng.standard_normal(n)
     res, dt = {}, None
-    for name, prec in (("fp64", cfdx.FP64), ("mixed", cfdx.MIXED)):
+    for name, prec in (("fp64", cfdx.FP64), ("mixed", cfdx.MIXED), ("fp64_perturbed", cfdx.FP64)):
         b = cfdx.Block(384, 128, 32, precision=prec, ctx=ctx, desc=cfdx.BlockDesc.from_buffer_copy(d))
         b.set_geometry_soa(geo)
         b.configure(0.5, 0.0, prm)
-        b.set_flow(flow0)
+        b.set_flow(flow_p if name == "fp64_perturbed" else flow0)
         if dt is None:
             dt = b.cfl_dt(0.5)
         b.configure(0.5, dt, prm)
@@ -288,13 +326,18 @@ def airfoil_c3(cfdx, steps: int):
         res[name] = (b.get_flow(), n * steps / (ms / 1e3), ms / steps)
         b.close()
     errs = field_errors(res["mixed"][0], res["fp64"][0])
+    sens = field_errors(res["fp64_perturbed"][0], res["fp64"][0])
     finite = all(bool(np.isfinite(cfdx_matrix(res[k][0])).all()) for k in res)
     return {"workload": "NACA 0012 C-grid 384x128x32, far field 20c, wall 1e-4c, Mach 0.5, AoA 2 deg, Re 1e5, MTS LES",
             "steps": steps, "dt": dt, "geometry_s": round(t_geo, 3),
             "fp64_value": res["fp64"][1], "mixed_value": res["mixed"][1], "unit": "cell-updates/s",
             "fp64_ms_per_step": res["fp64"][2], "mixed_ms_per_step": res["mixed"][2],
             "mixed_over_fp64": res["mixed"][1] / res["fp64"][1], "field_errors_mixed_vs_fp64": errs,
+            "fp64_sensitivity": {"perturbation": "u *= 1 + 1e-7 N(0,1) at t = 0", "field_errors_vs_fp64": sens},
             "state_finite": finite}
+
+
+SERVE_MAX_NODES = 5_000_000
 
 
 def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kind: str = "airfoil"):
@@ -312,10 +355,14 @@ def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kin
         out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, grids 128x64x16..384x128x32, "
                          "periodic TGV / vortex (curvilinear map), Mach U(0.1,0.6), Re U(500,5000)"}
     for name, prec, batched in (("batched_mixed", cfdx.MIXED, True), ("plain_fp64", cfdx.FP64, False)):
-        srv = serve.Server(precision=prec, batched=batched, chunk=10, graph=True, keep_flow=True)
+        # batched: at most SERVE_MAX_NODES nodes in flight (enough to fill the GPU; the earlier
+        # requests finish first under load)
+        srv = serve.Server(precision=prec, batched=batched, chunk=10, graph=True, keep_flow=True,
+                           max_nodes=SERVE_MAX_NODES if batched else None)
         res = srv.run(reqs, clock="wall")
         rep = serve.report(reqs, res)
         rep["peak_active"] = srv.peak_active
+        rep["max_nodes_in_flight"] = srv.max_nodes
         rep["peak_bytes"] = srv.peak_bytes
         rep["state_finite"] = all(bool(np.isfinite(cfdx_matrix(r.flow)).all()) for r in res.values())
         srv.close()
@@ -357,15 +404,14 @@ def main() -> int:
     cfdx = _import_pkg()
     spec, cdef, pkw, viscous = workload(args.workload)
     t_setup = time.perf_counter()
-    case = cfdx.periodic_case(cdef["ni"], cdef["nj"], cdef["nk"], **cdef["kw"])
+    # periodic workloads: the case built on the host (the same arrays the oracle would take)
+    case = None if "cgrid" in cdef else cfdx.periodic_case(cdef["ni"], cdef["nj"], cdef["nk"], **cdef["kw"])
     prm = cfdx.default_params(**pkw) if viscous else cfdx.euler_params()
     mach = cdef["kw"]["mach"]
-    dt = cfdx.cfl_dt(case, 0.5)
-    n = case["n"]
+    n = cdef["ni"] * cdef["nj"] * cdef["nk"]
     precision = cfdx.MIXED if args.precision == "mixed" else cfdx.FP64
     tiles = tuned_tiles(cfdx, cdef, precision, mach, prm) if args.tune else None
-    block = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=precision)
-    block.set_case(case)
+    block, dt = make_block(cfdx, cdef, precision, prm, host_case=case)
     if tiles:
         block.set_tiles(tuple(tiles["tiles"]))
     print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
@@ -422,7 +468,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if precision == cfdx.FP64 else "f32 fluxes / f64 state",
         "data": "synthetic",
-        "config": dict(spec, parallelism="single GPU, one periodic block", precision=args.precision),
+        "config": dict(spec, parallelism="single GPU, one block", precision=args.precision),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_face (i, j, k sweeps; mean per launch)", "kernel_ms": face_ms,
@@ -440,8 +486,8 @@ def main() -> int:
         # the other precision on the same initial state and step count: throughput and per-field error
         other = cfdx.FP64 if precision == cfdx.MIXED else cfdx.MIXED
         otiles = tuned_tiles(cfdx, cdef, other, mach, prm) if args.tune else None
-        ob = cfdx.Block(cdef["ni"], cdef["nj"], cdef["nk"], precision=other, ctx=ctx)
-        ob.set_case(case)
+        ob, _ = make_block(cfdx, cdef, other, prm, ctx=ctx, host_case=case)
+        ob.configure(mach, dt, prm)
         if otiles:
             ob.set_tiles(tuple(otiles["tiles"]))
         o_ms, o_tk = time_block(ob, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps)

@@ -186,6 +186,7 @@ void bc_extent(const BlockDev &B, LaunchList &L)
             L.bc_face = std::max(L.bc_face, f[d]);
         }
     if (B.off[1] && B.bc[2] == CFDX_BC_WALL_CUT) L.any_cut = true;
+    L.shell_max = std::max(L.shell_max, cfdx::blk::shell_count(B));
 }
 
 // single-block launch list (tile offsets 0 / total)
@@ -223,7 +224,8 @@ LaunchList single_list(cfdx_block *b)
 uint32_t mon_triples(const BlockDev &B, const cfdx::blk::TileCfg &t)
 {
     const int tx = cfdx::blk::NODE_TILES[t.node][0], ty = cfdx::blk::NODE_TILES[t.node][1];
-    return cfdx::blk::tiles_of(B, 0, t) * (uint32_t)(tx * ty / 64);
+    // node kernel waves, then one slot per wave of the shell kernel
+    return cfdx::blk::tiles_of(B, 0, t) * (uint32_t)(tx * ty / 64) + (cfdx::blk::shell_count(B) + 63) / 64;
 }
 
 int run_stage(cfdx_block *b, int mode)
@@ -297,6 +299,7 @@ int upload_batch(cfdx_batch *q)
     q->L.bc_face = 0;
     for (int d = 0; d < 3; ++d) q->L.bc_face_d[d] = 0;
     q->L.any_cut = false;
+    q->L.shell_max = 0;
     for (size_t i = 0; i < nb; ++i) bc_extent(h[i], q->L);
     q->seen.resize(nb);
     for (size_t i = 0; i < nb; ++i) q->seen[i] = q->blocks[i]->version;

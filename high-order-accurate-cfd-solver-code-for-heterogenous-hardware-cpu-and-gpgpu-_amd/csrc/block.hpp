@@ -58,6 +58,20 @@ struct BlockDev {
     cfdx::dev::ViscParams vp;
 };
 
+// Nodes of the shell of a bounded block (within 3 nodes of a side along a bounded direction; the
+// node kernels leave them to the shell kernels); 0 for a periodic block.
+__host__ __device__ inline uint32_t shell_count(const BlockDev &B)
+{
+    if (!B.code) return 0;
+    const uint32_t ni = B.ni, nj = B.nj, nk = B.nk;
+    const uint32_t kb = B.off[2] ? 3u : 0u, jb = B.off[1] ? 3u : 0u;
+    uint32_t n = 0;
+    if (B.off[2]) n += 6u * ni * nj;
+    if (B.off[1]) n += 6u * ni * (nk - 2 * kb);
+    if (B.off[0]) n += 6u * (nj - 2 * jb) * (nk - 2 * kb);
+    return n;
+}
+
 // One launch over a list of blocks.
 struct LaunchList {
     const BlockDev *blks = nullptr;   // device array
@@ -69,6 +83,7 @@ struct LaunchList {
     uint32_t bc_face = 0;             // largest side face (owned nodes) of the bounded blocks; 0 if none
     uint32_t bc_face_d[3] = {};       // the same per direction (0: no block bounded in that direction)
     bool any_cut = false;             // some block has a wake cut
+    uint32_t shell_max = 0;           // largest shell of the bounded blocks
 };
 
 // kernels (block_kernels.hip)

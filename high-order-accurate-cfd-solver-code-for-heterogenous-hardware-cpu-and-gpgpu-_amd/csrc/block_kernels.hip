@@ -82,6 +82,50 @@ __device__ __forceinline__ bool near_side(const BlockDev &B, int i, int j, int k
     return s || (B.off[d] && (c[d] < 3 || c[d] > n[d] - 4));
 }
 
+// The shell of a bounded block: the nodes within 3 of a side along a bounded direction (the only
+// nodes where a loc-selected form can differ from the central one). The node kernels skip it and
+// the shell kernels cover it, so the tiled kernels carry no loc branches. Enumeration: the k shell
+// planes (if k is bounded), then the j shell rows inside the k range left, then the i shell
+// columns inside the j and k ranges left; i runs fastest in the first two parts.
+__device__ __forceinline__ bool in_shell(const BlockDev &B, int i, int j, int k)
+{
+    const int c[3] = {i, j, k}, n[3] = {B.ni, B.nj, B.nk};
+    bool s = false;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s |= B.off[q] && (c[q] < 3 || c[q] > n[q] - 4);
+    return s;
+}
+
+__device__ __forceinline__ void shell_node(const BlockDev &B, uint32_t t, int &i, int &j, int &k)
+{
+    const uint32_t ni = B.ni, nj = B.nj, nk = B.nk;
+    const uint32_t kb = B.off[2] ? 3u : 0u, jb = B.off[1] ? 3u : 0u;
+    auto layer = [](uint32_t m, uint32_t n) { return (int)(m < 3 ? m : n - 6 + m); };
+    if (B.off[2]) {
+        const uint32_t part = 6u * ni * nj;
+        if (t < part) {
+            i = (int)(t % ni);
+            j = (int)((t / ni) % nj);
+            k = layer(t / (ni * nj), nk);
+            return;
+        }
+        t -= part;
+    }
+    if (B.off[1]) {
+        const uint32_t part = 6u * ni * (nk - 2 * kb);
+        if (t < part) {
+            i = (int)(t % ni);
+            j = layer((t / ni) % 6u, nj);
+            k = (int)(kb + t / (6u * ni));
+            return;
+        }
+        t -= part;
+    }
+    i = layer(t % 6u, ni);
+    j = (int)(jb + (t / 6u) % (nj - 2 * jb));
+    k = (int)(kb + t / (6u * (nj - 2 * jb)));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each
 // with its own L2. Remap the linear workgroup id so XCD x takes one contiguous range of tiles
 // (a slab of k planes of one block). Bijective for any count.
@@ -105,6 +149,72 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ pref
 // ---------------------------------------------------------------------------
 // viscous node kernel (Viscous.cl:285-631 on a periodic block: central 4th order)
 // ---------------------------------------------------------------------------
+// Fv, Ev, Gv (+ mu) of one node. LOC: a shell node of a bounded block (loc-selected forms near
+// the sides); otherwise the central forms.
+template <typename S, bool LOC>
+__device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k, int mode)
+{
+    const uint32_t g = node_index(B, i, j, k);
+    // stage vector into the viscous kernel: U0 at j = 0, the raw U1 / U2 after (Solver.cpp:427-435)
+    const double *const *Uin = (mode == 0 || mode == 3) ? B.U0 : B.Us;
+    double U[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) U[c] = ld(Uin[c], g);
+    double dU[5][3], KS[3];
+    const double iU0 = 1.0 / U[0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
+        const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+        double um1[5], up1[5];
+        if (LOC && near_side(B, i, j, k, d)) {
+            // bounded block: the loc-selected one-sided forms of Viscous.cl:285-435 near the sides
+            const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
+            const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
+                      LP2 = ld(B.code, p2);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double *u = Uin[c];
+                um1[c] = ld(u, m1);
+                up1[c] = ld(u, p1);
+                const double f[7] = {ld(u, m3), ld(u, m2), um1[c], U[c], up1[c], ld(u, p2), ld(u, p3)};
+                dU[c][d] = loc_derivative(f, LI, LM2, LM1, LP1, LP2, false);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double *u = Uin[c];
+                um1[c] = ld(u, m1);
+                up1[c] = ld(u, p1);
+                dU[c][d] = (1.0 / 12.0) * (8.0 * (up1[c] - um1[c]) - (ld(u, p2) - ld(u, m2)));
+            }
+        }
+        // MTS scale-similarity term of momentum component d+1 along d (Viscous.cl:493-495)
+        const double a = (U[1 + d] * iU0) - ((0.25 * um1[1 + d] + 0.5 * U[1 + d] + 0.25 * up1[1 + d]) / (0.25 * um1[0] + 0.5 * U[0] + 0.25 * up1[0]));
+        KS[d] = a * a;
+    }
+    // the differences above are taken in double; the flux arithmetic runs in S
+    S dUs[5][3], Uv[5], KSs[3], m[9];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        Uv[c] = S(U[c]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dUs[c][d] = S(dU[c][d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) KSs[d] = S(KS[d]);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) m[c] = S(ld(B.m[c], g));
+    S fv[3][5];
+    const S mu = viscous_node(dUs, Uv, ld((const S *)B.P[1], g), ld((const S *)B.P[2], g), ld((const S *)B.P[3], g), m,
+                              S(ld(B.det, g)), KSs, B.vp, fv);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 1; c < 5; ++c) st((S *)B.Fv[d][c - 1], g, fv[d][c]);
+    if (mode >= 2) st(B.mu, g, (double)mu);
+}
+
 template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                            uint32_t nblk, uint32_t total, int mode)
@@ -119,66 +229,22 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
         const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
         if (i >= B.ni || j >= B.nj) continue;
-        const uint32_t g = node_index(B, i, j, k);
-        // stage vector into the viscous kernel: U0 at j = 0, the raw U1 / U2 after (Solver.cpp:427-435)
-        const double *const *Uin = (mode == 0 || mode == 3) ? B.U0 : B.Us;
-        double U[5];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) U[c] = ld(Uin[c], g);
-        double dU[5][3], KS[3];
-        const double iU0 = 1.0 / U[0];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
-            const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
-            double um1[5], up1[5];
-            if (BND && B.code && near_side(B, i, j, k, d)) {
-                // bounded block: the loc-selected one-sided forms of Viscous.cl:285-435 near the sides
-                const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
-                const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
-                          LP2 = ld(B.code, p2);
-#pragma unroll
-                for (int c = 0; c < 5; ++c) {
-                    const double *u = Uin[c];
-                    um1[c] = ld(u, m1);
-                    up1[c] = ld(u, p1);
-                    const double f[7] = {ld(u, m3), ld(u, m2), um1[c], U[c], up1[c], ld(u, p2), ld(u, p3)};
-                    dU[c][d] = loc_derivative(f, LI, LM2, LM1, LP1, LP2, false);
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 5; ++c) {
-                    const double *u = Uin[c];
-                    um1[c] = ld(u, m1);
-                    up1[c] = ld(u, p1);
-                    dU[c][d] = (1.0 / 12.0) * (8.0 * (up1[c] - um1[c]) - (ld(u, p2) - ld(u, m2)));
-                }
-            }
-            // MTS scale-similarity term of momentum component d+1 along d (Viscous.cl:493-495)
-            const double a = (U[1 + d] * iU0) - ((0.25 * um1[1 + d] + 0.5 * U[1 + d] + 0.25 * up1[1 + d]) / (0.25 * um1[0] + 0.5 * U[0] + 0.25 * up1[0]));
-            KS[d] = a * a;
-        }
-        // the differences above are taken in double; the flux arithmetic runs in S
-        S dUs[5][3], Uv[5], KSs[3], m[9];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            Uv[c] = S(U[c]);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) dUs[c][d] = S(dU[c][d]);
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) KSs[d] = S(KS[d]);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) m[c] = S(ld(B.m[c], g));
-        S fv[3][5];
-        const S mu = viscous_node(dUs, Uv, ld((const S *)B.P[1], g), ld((const S *)B.P[2], g), ld((const S *)B.P[3], g), m,
-                                  S(ld(B.det, g)), KSs, B.vp, fv);
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int c = 1; c < 5; ++c) st((S *)B.Fv[d][c - 1], g, fv[d][c]);
-        if (mode >= 2) st(B.mu, g, (double)mu);
+        if (BND && B.code && in_shell(B, i, j, k)) continue;   // the shell kernel's node
+        visc_node<S, false>(B, i, j, k, mode);
     }
+}
+
+// the shell nodes of each bounded block of the list (blockIdx.y = block)
+template <typename S>
+__global__ void __launch_bounds__(256) k_visc_shell(const BlockDev *__restrict__ blks, int mode)
+{
+    const BlockDev &B = blks[blockIdx.y];
+    if (!B.viscous) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= shell_count(B)) return;
+    int i, j, k;
+    shell_node(B, t, i, j, k);
+    visc_node<S, true>(B, i, j, k, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -284,6 +350,107 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 // update kernel: L and the SSP-RK3 stage (Weno.cl:3060-3135), in double.
 //   mode 0..2: RK stage j; mode 3: write L to the block's Lout only
 // ---------------------------------------------------------------------------
+// L and the SSP-RK3 stage of one node; mdu / mdv / mde: the node's monitor values (mode 2 with
+// the monitor on). LOC as in visc_node.
+template <typename S, bool LOC>
+__device__ __forceinline__ void update_node(const BlockDev &B, int i, int j, int k, int mode, double &mdu, double &mdv,
+                                            double &mde)
+{
+    const uint32_t g = node_index(B, i, j, k);
+    double L[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t m1 = shifted(B, i, j, k, d, -1);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const S *fh = (const S *)B.FH[d][c];
+            L[c] += (double)ld(fh, g) - (double)ld(fh, m1);
+        }
+    }
+    if (B.viscous) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
+            const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
+            if (LOC && near_side(B, i, j, k, d)) {
+                // bounded block: the loc-selected forms of Weno.cl:2732-2856
+                const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
+                const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
+                          LP2 = ld(B.code, p2);
+#pragma unroll
+                for (int c = 1; c < 5; ++c) {
+                    const S *fv = (const S *)B.Fv[d][c - 1];
+                    const double f[7] = {(double)ld(fv, m3), (double)ld(fv, m2), (double)ld(fv, m1), (double)ld(fv, g),
+                                         (double)ld(fv, p1), (double)ld(fv, p2), (double)ld(fv, p3)};
+                    L[c] += loc_derivative(f, LI, LM2, LM1, LP1, LP2, true);
+                }
+            } else {
+#pragma unroll
+                for (int c = 1; c < 5; ++c) {
+                    const S *f = (const S *)B.Fv[d][c - 1];
+                    L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) L[c] = -L[c];
+    if (mode == 3) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) B.Lout[(size_t)c * B.n + g] = L[c];
+    } else {
+        const double del_t = B.del_t;
+        double un[5], u0[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) u0[c] = ld(B.U0[c], g);
+        if (mode == 0) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) { un[c] = u0[c] + del_t * L[c]; st(B.Us[c], g, un[c]); }
+        } else if (mode == 1) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) { un[c] = 0.75 * u0[c] + 0.25 * ld(B.Us[c], g) + 0.25 * del_t * L[c]; st(B.Us[c], g, un[c]); }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) un[c] = (u0[c] * (1.0 / 3.0)) + ((2.0 / 3.0) * ld(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
+        }
+        const double det = ld(B.det, g);
+        const double idet = 1.0 / det;
+        const double rho = idet * un[0], ir = 1.0 / rho;
+        const double u = (idet * un[1]) * ir, v = (idet * un[2]) * ir, w = (idet * un[3]) * ir;
+        const double e = (idet * un[4]) * ir - 0.5 * (u * u + v * v + w * w);
+        const double p = e * rho * 0.4;
+        st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
+        if (mode == 2) {
+            // next step's RK base: U_Q recomputes det*Q from the new primitives (Solver.cpp:377-388)
+            double q[5];
+            prim_to_q(rho, u, v, w, p, q);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
+            if (B.mon) {
+                const double r0 = idet * u0[0], ir0 = 1.0 / r0;
+                const double uo = (idet * u0[1]) * ir0, vo = (idet * u0[2]) * ir0, wo = (idet * u0[3]) * ir0;
+                const double eo = (idet * u0[4]) * ir0 - 0.5 * (uo * uo + vo * vo + wo * wo);
+                mdu = fabs(u - uo); mdv = fabs(v - vo); mde = fabs(e - eo);
+            }
+        }
+    }
+}
+
+// one maximum per wave of the monitor values (every lane takes part), written by lane 0 at slot
+__device__ __forceinline__ void monitor_wave(const BlockDev &B, uint32_t slot, double mdu, double mdv, double mde)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mdu = fmax(mdu, __shfl_xor(mdu, o));
+        mdv = fmax(mdv, __shfl_xor(mdv, o));
+        mde = fmax(mde, __shfl_xor(mde, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        double *m = B.mon + 3 * (size_t)slot;
+        m[0] = mdu; m[1] = mdv; m[2] = mde;
+    }
+}
+
 template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                    uint32_t nblk, uint32_t total, int mode)
@@ -298,99 +465,30 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
         const int k = (int)(lt / (ntx * nty));
         // residual monitor of the step (Solver.cpp:546-573): |du|, |dv|, |de| against the RK base
         double mdu = 0.0, mdv = 0.0, mde = 0.0;
-        if (i < B.ni && j < B.nj) {
-            const uint32_t g = node_index(B, i, j, k);
-            double L[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const uint32_t m1 = shifted(B, i, j, k, d, -1);
-#pragma unroll
-                for (int c = 0; c < 5; ++c) {
-                    const S *fh = (const S *)B.FH[d][c];
-                    L[c] += (double)ld(fh, g) - (double)ld(fh, m1);
-                }
-            }
-            if (B.viscous) {
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
-                    const uint32_t p1 = shifted(B, i, j, k, d, 1), p2 = shifted(B, i, j, k, d, 2);
-                    if (BND && B.code && near_side(B, i, j, k, d)) {
-                        // bounded block: the loc-selected forms of Weno.cl:2732-2856
-                        const uint32_t m3 = shifted(B, i, j, k, d, -3), p3 = shifted(B, i, j, k, d, 3);
-                        const int LI = ld(B.code, g), LM2 = ld(B.code, m2), LM1 = ld(B.code, m1), LP1 = ld(B.code, p1),
-                                  LP2 = ld(B.code, p2);
-#pragma unroll
-                        for (int c = 1; c < 5; ++c) {
-                            const S *fv = (const S *)B.Fv[d][c - 1];
-                            const double f[7] = {(double)ld(fv, m3), (double)ld(fv, m2), (double)ld(fv, m1), (double)ld(fv, g),
-                                                 (double)ld(fv, p1), (double)ld(fv, p2), (double)ld(fv, p3)};
-                            L[c] += loc_derivative(f, LI, LM2, LM1, LP1, LP2, true);
-                        }
-                    } else {
-#pragma unroll
-                        for (int c = 1; c < 5; ++c) {
-                            const S *f = (const S *)B.Fv[d][c - 1];
-                            L[c] += (1.0 / 12.0) * (8.0 * ((double)ld(f, p1) - (double)ld(f, m1)) - ((double)ld(f, p2) - (double)ld(f, m2)));
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 5; ++c) L[c] = -L[c];
-            if (mode == 3) {
-#pragma unroll
-                for (int c = 0; c < 5; ++c) B.Lout[(size_t)c * B.n + g] = L[c];
-            } else {
-                const double del_t = B.del_t;
-                double un[5], u0[5];
-#pragma unroll
-                for (int c = 0; c < 5; ++c) u0[c] = ld(B.U0[c], g);
-                if (mode == 0) {
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) { un[c] = u0[c] + del_t * L[c]; st(B.Us[c], g, un[c]); }
-                } else if (mode == 1) {
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) { un[c] = 0.75 * u0[c] + 0.25 * ld(B.Us[c], g) + 0.25 * del_t * L[c]; st(B.Us[c], g, un[c]); }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) un[c] = (u0[c] * (1.0 / 3.0)) + ((2.0 / 3.0) * ld(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
-                }
-                const double det = ld(B.det, g);
-                const double idet = 1.0 / det;
-                const double rho = idet * un[0], ir = 1.0 / rho;
-                const double u = (idet * un[1]) * ir, v = (idet * un[2]) * ir, w = (idet * un[3]) * ir;
-                const double e = (idet * un[4]) * ir - 0.5 * (u * u + v * v + w * w);
-                const double p = e * rho * 0.4;
-                st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
-                if (mode == 2) {
-                    // next step's RK base: U_Q recomputes det*Q from the new primitives (Solver.cpp:377-388)
-                    double q[5];
-                    prim_to_q(rho, u, v, w, p, q);
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
-                    if (B.mon) {
-                        const double r0 = idet * u0[0], ir0 = 1.0 / r0;
-                        const double uo = (idet * u0[1]) * ir0, vo = (idet * u0[2]) * ir0, wo = (idet * u0[3]) * ir0;
-                        const double eo = (idet * u0[4]) * ir0 - 0.5 * (uo * uo + vo * vo + wo * wo);
-                        mdu = fabs(u - uo); mdv = fabs(v - vo); mde = fabs(e - eo);
-                    }
-                }
-            }
-        }
-        if (mode == 2 && B.mon) {
-            // one maximum per wave (every lane takes part), written by lane 0
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                mdu = fmax(mdu, __shfl_xor(mdu, o));
-                mdv = fmax(mdv, __shfl_xor(mdv, o));
-                mde = fmax(mde, __shfl_xor(mde, o));
-            }
-            if ((threadIdx.x & 63) == 0) {
-                double *m = B.mon + 3 * ((size_t)lt * (TX * TY / 64) + threadIdx.x / 64);
-                m[0] = mdu; m[1] = mdv; m[2] = mde;
-            }
-        }
+        if (i < B.ni && j < B.nj && !(BND && B.code && in_shell(B, i, j, k)))
+            update_node<S, false>(B, i, j, k, mode, mdu, mdv, mde);
+        if (mode == 2 && B.mon) monitor_wave(B, lt * (TX * TY / 64) + threadIdx.x / 64, mdu, mdv, mde);
+    }
+}
+
+// the shell nodes of each bounded block of the list (blockIdx.y = block); their monitor slots
+// follow the node kernel's (tiles x waves per tile, wpt), one per wave of 64 shell nodes
+template <typename S>
+__global__ void __launch_bounds__(256) k_update_shell(const BlockDev *__restrict__ blks, int mode, int tx, int ty)
+{
+    const BlockDev &B = blks[blockIdx.y];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ns = shell_count(B);
+    if (blockIdx.x * blockDim.x >= ns) return;   // whole workgroup past the shell
+    double mdu = 0.0, mdv = 0.0, mde = 0.0;
+    if (t < ns) {
+        int i, j, k;
+        shell_node(B, t, i, j, k);
+        update_node<S, true>(B, i, j, k, mode, mdu, mdv, mde);
+    }
+    if (mode == 2 && B.mon && (t & ~63u) < ns) {
+        const uint32_t main_slots = ((B.ni + tx - 1) / tx) * ((B.nj + ty - 1) / ty) * B.nk * (uint32_t)(tx * ty / 64);
+        monitor_wave(B, main_slots + t / 64, mdu, mdv, mde);
     }
 }
 
@@ -632,8 +730,14 @@ template <typename S, int V, bool BND>
 void launch_node_b(const LaunchList &L, dim3 g, hipStream_t st, bool visc_kernel, int mode)
 {
     constexpr int TX = NODE_TILES[V][0], TY = NODE_TILES[V][1];
-    if (visc_kernel) hipLaunchKernelGGL((k_visc<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
-    else hipLaunchKernelGGL((k_update<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+    const dim3 gs((L.shell_max + 255) / 256, L.nblk);
+    if (visc_kernel) {
+        hipLaunchKernelGGL((k_visc<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+        if (BND) hipLaunchKernelGGL(k_visc_shell<S>, gs, dim3(256), 0, st, L.blks, mode);
+    } else {
+        hipLaunchKernelGGL((k_update<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
+        if (BND) hipLaunchKernelGGL(k_update_shell<S>, gs, dim3(256), 0, st, L.blks, mode, TX, TY);
+    }
 }
 // lists without bounded blocks run the node kernels without the loc-code branches
 template <typename S, int V>

@@ -3,8 +3,9 @@
 Each request is one block (grid size, map, initial state, Mach, Re, number of RK3 steps). The
 server keeps the state of every admitted request resident in HBM and steps all of them in the
 same batched launches (continuous batching): between chunks of steps it retires finished
-requests (result copied to the host, block freed) and admits queued ones while the device has
-free memory (hipMemGetInfo minus a reserve, optionally capped by a byte budget). A request's
+requests (result copied to the host, block freed) and admits queued ones (first come, first
+served) while the device has free memory (hipMemGetInfo minus a reserve, optionally capped by a
+byte budget) and, optionally, while the nodes in flight stay under a cap. A request's
 arithmetic does not depend on what else is in the batch, so its result is the same alone and
 in the mix (the tests check bitwise equality).
 
@@ -125,11 +126,15 @@ def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) ->
 class Server:
     def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
                  budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
-                 grid_cap: int = 0, batched: bool = True, keep_flow: bool = True, prep_workers: int = 8):
+                 grid_cap: int = 0, batched: bool = True, keep_flow: bool = True, prep_workers: int = 8,
+                 max_nodes: int | None = None):
         self.ctx = ctx or Ctx()
         self.precision = precision
         self.reserve = reserve_bytes
         self.budget = budget_bytes
+        # in-flight node cap: enough nodes to fill the GPU, so that under load the earlier requests
+        # finish first instead of every admitted request sharing the device until the end
+        self.max_nodes = max_nodes
         self.max_active = max_active if batched else 1
         self.chunk = max(1, chunk)
         self.graph = graph
@@ -142,7 +147,9 @@ class Server:
         self.chunks = 0
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, prep_workers))
 
-    def _fits(self, req: Request, used: int) -> bool:
+    def _fits(self, req: Request, used: int, nodes: int = 0, busy: bool = False) -> bool:
+        if self.max_nodes is not None and busy and nodes + req.nodes > self.max_nodes:
+            return False
         need = req.bytes(self.precision)
         if self.budget is not None and used + need > self.budget:
             return False
@@ -173,7 +180,8 @@ class Server:
                     preps[req.rid] = self.pool.submit(prepare, req)
             # admission: FIFO while memory and slots allow, once the host set-up is ready (it is
             # waited for when nothing else could run)
-            while queue and len(active) < self.max_active and self._fits(queue[0], used):
+            while queue and len(active) < self.max_active and self._fits(
+                    queue[0], used, sum(a["req"].nodes for a in active.values()), bool(active)):
                 req = queue[0]
                 fut = preps.get(req.rid)
                 if fut is not None and not fut.done() and active:

@@ -5,8 +5,8 @@ tile shapes (`cfdx.tile_variants`). The tuner times them on the actual grid with
 the block's stream and keeps the fastest: a coordinate search, one kernel class at a time with
 the others fixed at the best found so far (the classes are separate launches, so their times
 are nearly independent). For a batch the measure is the whole batched step, and the persistent
-grid size is tuned too. Results are cached in a JSON table keyed by grid shape, precision and
-the viscous switch (`CFDX_TILE_CACHE`, default ~/.cache/cfdx/tiles.json).
+grid size is tuned too. Results are cached in a JSON table keyed by grid shape, precision, the
+viscous switch and the periodic mask (`CFDX_TILE_CACHE`, default ~/.cache/cfdx/tiles.json).
 
 Tuning runs real RK3 steps: the state of the block or batch advances by them. Tune a scratch
 copy (same shape) when the state matters.
@@ -51,8 +51,10 @@ class TileCache:
                 pass
 
 
-def block_key(ni: int, nj: int, nk: int, precision: int, viscous: bool) -> str:
-    return f"block:{ni}x{nj}x{nk}:p{precision}:v{int(bool(viscous))}"
+def block_key(ni: int, nj: int, nk: int, precision: int, viscous: bool, periodic: int = 7) -> str:
+    """Cache key of a block: shape, precision, viscous switch and (bounded blocks) the periodic mask."""
+    tail = "" if periodic == 7 else f":per{periodic}"
+    return f"block:{ni}x{nj}x{nk}:p{precision}:v{int(bool(viscous))}{tail}"
 
 
 def _counts() -> list[int]:
@@ -62,7 +64,7 @@ def _counts() -> list[int]:
 def tune_block(block: Block, mach: float, dt: float, params, steps: int = 2, cache: TileCache | None = None,
                force: bool = False) -> dict:
     """Chooses and sets the tile variants of one block; returns {"tiles", "times_ms", "cached"}."""
-    key = block_key(block.desc.ni, block.desc.nj, block.desc.nk, block.precision, params.viscous)
+    key = block_key(block.desc.ni, block.desc.nj, block.desc.nk, block.precision, params.viscous, block.desc.periodic)
     if cache is not None and not force:
         hit = cache.get(key)
         if hit:

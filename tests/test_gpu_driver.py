@@ -46,6 +46,45 @@ def test_monitor_matches_host_difference(ctx, precision):
             assert abs(got - w) <= tol * w, (got, w)
 
 
+def test_monitor_bounded_block(ctx):
+    """A walled box: the shell kernels' monitor slots take part. The residual covers every owned node
+    (side nodes with their value before the side conditions), so it is at least the host difference
+    over the nodes the side conditions leave alone; it does not depend on the node tile shape."""
+    prm = cfdx.default_params(mach=0.4, reyl=800.0)
+    d = cfdx.BlockDesc(14, 12, 10, 0, cfdx.FP64)
+    for q, bc in enumerate((1, 3, 1, 2, 4, 4)):
+        d.bc[q] = bc
+    xyz, geo = cfdx.box_geometry(d, (1.3, 1.1, 0.9), 0.02, (0.3, 1.2, 2.1))
+    own = cfdx.owned_index(d)
+    x, y = xyz[own, 0], xyz[own, 1]
+    flow = cfdx.freestream(own.size, 0.4, 0.0)
+    flow["u"] *= 1 + 0.2 * np.exp(-((y - 0.15) ** 2) / 0.01) * np.sin(3 * x)   # a bump near the j- wall
+    res = []
+    for tiles in ((0, 0, 0, 0), (0, 0, 0, 3)):
+        b = cfdx.Block(14, 12, 10, ctx=ctx, desc=cfdx.BlockDesc.from_buffer_copy(d))
+        b.set_geometry_soa(geo)
+        b.configure(0.4, 0.0, prm)
+        b.set_flow(flow)
+        dt = 0.5 * b.cfl_dt(0.5)
+        b.set_tiles(tiles)
+        b.set_monitor(True)
+        before = b.get_flow()
+        b.step(1, 0.4, dt, prm)
+        after = b.get_flow()
+        res.append(b.residual())
+        b.close()
+    assert res[0] == res[1]
+    dims, off = cfdx.padded(d)
+    i, j, k = (own % dims[0]) - off[0], ((own // dims[0]) % dims[1]) - off[1], (own // (dims[0] * dims[1])) - off[2]
+    inner = (i > 0) & (i < 13) & (j > 0) & (j < 11) & (k > 0) & (k < 9)
+    e0 = before["p"] / (0.4 * before["rho"])
+    e1 = after["p"] / (0.4 * after["rho"])
+    host = (np.abs(after["u"] - before["u"])[inner].max(), np.abs(after["v"] - before["v"])[inner].max(),
+            np.abs(e1 - e0)[inner].max())
+    for got, w in zip(res[0], host):
+        assert np.isfinite(got) and got >= w * (1 - 1e-9), (got, w)
+
+
 def test_driver_restart_resume(ctx, tmp_path):
     prm = cfdx.default_params(mach=0.3, reyl=800.0)
     ref = tgv(ctx)
