@@ -38,6 +38,26 @@ VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 ALG_BYTES_PER_CELL_UPDATE = 848  # minimal FP64 SoA traffic of one RK3 step (3 stages)
 # k_face per face: reads the 5 primitives P and writes the 5 face fluxes F-hat in the flux type
 # (8 B FP64, 4 B mixed) and reads 8 FP64 geometry values (metric row 3, det, face normal 3, face det)
+# Operations per node written out in the node kernels (counted from block_kernels.hip; the device
+# physics they call is counted by cfdx_op_count): viscous kernel stencil differences and MTS
+# similarity terms; update kernel face-flux differences, viscous divergence, mean RK stage and
+# primitive recovery.
+VISC_KERNEL_OPS = 118
+UPDATE_KERNEL_OPS_EULER = 30 + 70.0 / 3 + 20   # 70 / 3: the mean RK stage, 5 x {2, 6, 6} over the 3 stages
+UPDATE_KERNEL_OPS_VISC_DIV = 72
+
+
+def alg_ops_per_cell_update(counts: dict, viscous: bool) -> float:
+    """Algorithmic operations of one node over one SSP-RK3 step: per stage three faces, three node
+    fluxes, the conserved state and (viscous) the viscous node fluxes, plus the update kernel's
+    arithmetic; at the last stage the RK base det * Q."""
+    stage = 3 * counts["face"]["total"] + 3 * counts["node_flux"]["total"] + counts["cons"]["total"]
+    stage += UPDATE_KERNEL_OPS_EULER
+    if viscous:
+        stage += counts["viscous"]["total"] + VISC_KERNEL_OPS + UPDATE_KERNEL_OPS_VISC_DIV
+    return 3 * stage + counts["cons"]["total"] + 5
+
+
 def face_bytes_per_node(precision: str) -> int:
     return 64 + 10 * (8 if precision == "fp64" else 4)
 
@@ -455,6 +475,15 @@ def main() -> int:
                 "counts_from": prof["source"], "profile_rows": prof["rows"], "same_variants": prof["same_variants"],
                 "profiled_kernel_us": prof["avg_us"]}
 
+    counts = ctx.op_count(prm)
+    ops = alg_ops_per_cell_update(counts, viscous)
+    alg = {"ops_per_cell_update": ops, "ops_per_face": counts["face"]["total"],
+           "counts": {k: v for k, v in counts.items()},
+           "achieved_tops": ops * value / 1e12, "peak_tops": FP64_PEAK_TFLOPS,
+           "frac": ops * value / (FP64_PEAK_TFLOPS * 1e12),
+           "note": "static count of the kernels' arithmetic (cfdx_op_count + the node kernels' own lines); "
+                   "divisions, square roots and reciprocals count one each"}
+
     line = {
         "metric": "cell-updates/s",
         "value": value,
@@ -475,6 +504,7 @@ def main() -> int:
                      "bytes_per_launch": face_bytes * n},
         "valu": valu,
         "hbm_alg_frac_whole_step": ALG_BYTES_PER_CELL_UPDATE * value / (HBM_PEAK_GBS * 1e9),
+        "valu_alg_whole_step": alg,
         "kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in tk.items() if k.endswith("_ms")},
         "state_finite": finite,
         "copy_kernel": {"bytes_moved": 2 * nb, "ms": copy_ms, "GBs": 2 * nb / (copy_ms / 1e3) / 1e9},

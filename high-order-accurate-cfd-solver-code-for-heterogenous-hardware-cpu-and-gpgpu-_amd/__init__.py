@@ -143,6 +143,7 @@ EXPORTS = {
     "cfdx_block_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_block_get_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_batch_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
+    "cfdx_op_count": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(_i64)]),
     "cfdx_block_set_monitor": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_residual": (C.c_int, [_vp, C.POINTER(_d)]),
     "cfdx_write_restart": (C.c_int, [C.c_char_p, _i32, _i64, _vp, _i64, _i32]),
@@ -401,6 +402,18 @@ class Ctx:
         f, t = _i64(0), _i64(0)
         _check(_lib.cfdx_mem_info(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+    def op_count(self, params=None) -> dict:
+        """Static operation count of the scheme's arithmetic per face / node (cfdx_op_count):
+        {group: {add, mul, div, sqrt, rcp, cbrt, total}} for groups face, cons, node_flux, viscous."""
+        out = (_i64 * 24)()
+        _check(_lib.cfdx_op_count(self.h, C.byref(params) if params is not None else None, out))
+        res = {}
+        for g, name in enumerate(("face", "cons", "node_flux", "viscous")):
+            d = dict(zip(("add", "mul", "div", "sqrt", "rcp", "cbrt"), out[6 * g:6 * g + 6]))
+            d["total"] = sum(d.values())
+            res[name] = d
+        return res
 
     def record(self, slot: int) -> None:
         _check(_lib.cfdx_event_record(self.h, slot))

@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace cfdx {
 namespace dev {
 
@@ -149,7 +151,7 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     const T ISm0 = c13 * s2 * s2 + c14 * u0 * u0;
     const T ISm1 = c13 * s1 * s1 + c14 * u1 * u1;
     const T ISm2 = c13 * s0 * s0 + c14 * u2 * u2;
-    if constexpr (sizeof(T) == sizeof(double)) {
+    if constexpr (!std::is_same<T, float>::value) {
         // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2: one division serves both sides
         T np_, dp_, nn_, dn_;
         {

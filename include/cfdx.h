@@ -256,6 +256,14 @@ int cfdx_block_get_tiles(cfdx_block *b, cfdx_tiles *t);
 /* Batches: blocks of different sizes and run parameters stepped together. Each kernel class is
  * one launch over the tiles of every block; a step can be replayed from a captured hipGraph.
  * A batch references its blocks (it does not own them); all share its precision and device. */
+/* Static operation count of the scheme's arithmetic (no counterpart in the reference; for
+ * measurement): the device physics run once on a counting number type. out[6 g + c], groups g =
+ * 0 one face of a face sweep, 1 the conserved state of a node, 2 a node flux along one metric row,
+ * 3 the viscous + MTS node fluxes of a node; counts c = add/sub, mul, div, sqrt (incl. 1/sqrt),
+ * reciprocal, cube root (a fused multiply-add counts as one add and one mul). The arithmetic
+ * written out in the node kernels themselves is not included (see opcount.hip). */
+int cfdx_op_count(cfdx_ctx *ctx, const cfdx_params *params, int64_t out[24]);
+
 typedef struct cfdx_batch cfdx_batch;
 int cfdx_batch_create(cfdx_ctx *ctx, int precision, cfdx_batch **out);
 int cfdx_batch_destroy(cfdx_batch *q);
