@@ -215,11 +215,25 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
     if (mode >= 2) st(B.mu, g, (double)mu);
 }
 
+// Node tiles of the list first (tile ids through the XCD remap), then, for lists with bounded
+// blocks, nblk x shell_tiles shell tiles of TX * TY shell nodes each (block = id / shell_tiles),
+// dispatched last in plain order so that they spread over the XCDs at the end of the launch.
 template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
-                                                           uint32_t nblk, uint32_t total, int mode)
+                                                           uint32_t nblk, uint32_t total, int mode, uint32_t shell_tiles)
 {
-    for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const uint32_t all = total + (BND ? nblk * shell_tiles : 0u);
+    for (uint32_t wg = blockIdx.x; wg < all; wg += gridDim.x) {
+        if (BND && wg >= total) {
+            const uint32_t s = wg - total;
+            const BlockDev &B = blks[s / shell_tiles];
+            const uint32_t t = (s % shell_tiles) * (TX * TY) + threadIdx.x;
+            if (!B.viscous || t >= shell_count(B)) continue;
+            int i, j, k;
+            shell_node(B, t, i, j, k);
+            visc_node<S, true>(B, i, j, k, mode);
+            continue;
+        }
         const uint32_t t = xcd_tile(wg, total);
         const BlockDev &B = blks[find_block(prefix, nblk, t)];
         if (!B.viscous) continue;
@@ -229,22 +243,9 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
         const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
         if (i >= B.ni || j >= B.nj) continue;
-        if (BND && B.code && in_shell(B, i, j, k)) continue;   // the shell kernel's node
+        if (BND && B.code && in_shell(B, i, j, k)) continue;   // a shell tile's node
         visc_node<S, false>(B, i, j, k, mode);
     }
-}
-
-// the shell nodes of each bounded block of the list (blockIdx.y = block)
-template <typename S>
-__global__ void __launch_bounds__(256) k_visc_shell(const BlockDev *__restrict__ blks, int mode)
-{
-    const BlockDev &B = blks[blockIdx.y];
-    if (!B.viscous) return;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= shell_count(B)) return;
-    int i, j, k;
-    shell_node(B, t, i, j, k);
-    visc_node<S, true>(B, i, j, k, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,11 +452,33 @@ __device__ __forceinline__ void monitor_wave(const BlockDev &B, uint32_t slot, d
     }
 }
 
+// tiles as in k_visc; the shell tiles' monitor slots follow the block's node tiles' (tiles x
+// waves per tile), one per wave of 64 shell nodes
 template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
-                                                   uint32_t nblk, uint32_t total, int mode)
+                                                   uint32_t nblk, uint32_t total, int mode, uint32_t shell_tiles)
 {
-    for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const uint32_t all = total + (BND ? nblk * shell_tiles : 0u);
+    for (uint32_t wg = blockIdx.x; wg < all; wg += gridDim.x) {
+        // residual monitor of the step (Solver.cpp:546-573): |du|, |dv|, |de| against the RK base
+        double mdu = 0.0, mdv = 0.0, mde = 0.0;
+        if (BND && wg >= total) {
+            const uint32_t s = wg - total;
+            const BlockDev &B = blks[s / shell_tiles];
+            const uint32_t t = (s % shell_tiles) * (TX * TY) + threadIdx.x;
+            const uint32_t ns = shell_count(B);
+            if ((s % shell_tiles) * (TX * TY) >= ns) continue;   // whole tile past the shell
+            if (t < ns) {
+                int i, j, k;
+                shell_node(B, t, i, j, k);
+                update_node<S, true>(B, i, j, k, mode, mdu, mdv, mde);
+            }
+            if (mode == 2 && B.mon && (t & ~63u) < ns) {
+                const uint32_t main_slots = ((B.ni + TX - 1) / TX) * ((B.nj + TY - 1) / TY) * B.nk * (uint32_t)(TX * TY / 64);
+                monitor_wave(B, main_slots + t / 64, mdu, mdv, mde);
+            }
+            continue;
+        }
         const uint32_t t = xcd_tile(wg, total);
         const BlockDev &B = blks[find_block(prefix, nblk, t)];
         const uint32_t lt = t - B.tile0[0];
@@ -463,32 +486,9 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
         const int i = (int)((lt % ntx) * TX + threadIdx.x % TX);
         const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
-        // residual monitor of the step (Solver.cpp:546-573): |du|, |dv|, |de| against the RK base
-        double mdu = 0.0, mdv = 0.0, mde = 0.0;
         if (i < B.ni && j < B.nj && !(BND && B.code && in_shell(B, i, j, k)))
             update_node<S, false>(B, i, j, k, mode, mdu, mdv, mde);
         if (mode == 2 && B.mon) monitor_wave(B, lt * (TX * TY / 64) + threadIdx.x / 64, mdu, mdv, mde);
-    }
-}
-
-// the shell nodes of each bounded block of the list (blockIdx.y = block); their monitor slots
-// follow the node kernel's (tiles x waves per tile, wpt), one per wave of 64 shell nodes
-template <typename S>
-__global__ void __launch_bounds__(256) k_update_shell(const BlockDev *__restrict__ blks, int mode, int tx, int ty)
-{
-    const BlockDev &B = blks[blockIdx.y];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t ns = shell_count(B);
-    if (blockIdx.x * blockDim.x >= ns) return;   // whole workgroup past the shell
-    double mdu = 0.0, mdv = 0.0, mde = 0.0;
-    if (t < ns) {
-        int i, j, k;
-        shell_node(B, t, i, j, k);
-        update_node<S, true>(B, i, j, k, mode, mdu, mdv, mde);
-    }
-    if (mode == 2 && B.mon && (t & ~63u) < ns) {
-        const uint32_t main_slots = ((B.ni + tx - 1) / tx) * ((B.nj + ty - 1) / ty) * B.nk * (uint32_t)(tx * ty / 64);
-        monitor_wave(B, main_slots + t / 64, mdu, mdv, mde);
     }
 }
 
@@ -730,14 +730,16 @@ template <typename S, int V, bool BND>
 void launch_node_b(const LaunchList &L, dim3 g, hipStream_t st, bool visc_kernel, int mode)
 {
     constexpr int TX = NODE_TILES[V][0], TY = NODE_TILES[V][1];
-    const dim3 gs((L.shell_max + 255) / 256, L.nblk);
-    if (visc_kernel) {
-        hipLaunchKernelGGL((k_visc<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
-        if (BND) hipLaunchKernelGGL(k_visc_shell<S>, gs, dim3(256), 0, st, L.blks, mode);
-    } else {
-        hipLaunchKernelGGL((k_update<S, TX, TY, BND>), g, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode);
-        if (BND) hipLaunchKernelGGL(k_update_shell<S>, gs, dim3(256), 0, st, L.blks, mode, TX, TY);
-    }
+    static_assert(TX * TY == 256, "node tiles have 256 threads");
+    const uint32_t shell_tiles = BND ? (L.shell_max + TX * TY - 1) / (TX * TY) : 0u;
+    dim3 ga = g;
+    if (BND && !L.grid_cap) ga.x += L.nblk * shell_tiles;   // one workgroup per shell tile too
+    if (visc_kernel)
+        hipLaunchKernelGGL((k_visc<S, TX, TY, BND>), ga, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode,
+                           shell_tiles);
+    else
+        hipLaunchKernelGGL((k_update<S, TX, TY, BND>), ga, dim3(TX * TY), 0, st, L.blks, L.prefix[0], L.nblk, L.total[0], mode,
+                           shell_tiles);
 }
 // lists without bounded blocks run the node kernels without the loc-code branches
 template <typename S, int V>
