@@ -525,8 +525,10 @@ class Block:
         if desc is None:
             desc = BlockDesc(ni, nj, nk, 7, precision)
         else:
+            desc = BlockDesc.from_buffer_copy(desc)
             desc.precision = precision
-            assert (desc.ni, desc.nj, desc.nk) == (ni, nj, nk)
+            if (desc.ni, desc.nj, desc.nk) != (ni, nj, nk):
+                raise ValueError(f"descriptor {desc.ni}x{desc.nj}x{desc.nk} does not match the block {ni}x{nj}x{nk}")
         self.desc = desc
         h = _vp()
         _check(_lib.cfdx_block_create(self.ctx.h, C.byref(self.desc), C.byref(h)))
@@ -543,12 +545,16 @@ class Block:
         if desc is not None:
             d = BlockDesc.from_buffer_copy(desc)
             d.precision = precision
-        return int(_lib.cfdx_block_bytes(C.byref(d)))
+        n = int(_lib.cfdx_block_bytes(C.byref(d)))
+        if n < 0:
+            raise CfdxError(n, _lib.cfdx_last_error().decode())
+        return n
 
     def set_geometry_soa(self, geo: np.ndarray) -> None:
         dims, _ = padded(self.desc)
         geo = np.ascontiguousarray(geo, dtype=np.float64)
-        assert geo.size == 22 * int(np.prod(dims))
+        if geo.size != 22 * int(np.prod(dims)):
+            raise ValueError(f"geometry holds {geo.size} values, the block needs 22 x {int(np.prod(dims))}")
         _check(_lib.cfdx_block_set_geometry_soa(self.h, _ptr(geo)))
 
     def close(self) -> None:
@@ -588,7 +594,8 @@ class Block:
 
     def set_flow(self, flow: np.ndarray) -> None:
         flow = np.ascontiguousarray(flow)
-        assert flow.dtype == FLOW_DT and flow.shape[0] == self.n
+        if flow.dtype != FLOW_DT or flow.shape[0] != self.n:
+            raise ValueError(f"flow must be a FLOW_DT array of {self.n} owned nodes")
         _check(_lib.cfdx_block_set_flow(self.h, _ptr(flow)))
 
     def get_flow(self) -> np.ndarray:
@@ -687,7 +694,8 @@ class Batch:
 # ---------------------------------------------------------------------------
 def write_restart(directory: str, slot: int, flow: np.ndarray, iteration: int, digits: int = 17) -> None:
     flow = np.ascontiguousarray(flow)
-    assert flow.dtype == FLOW_DT
+    if flow.dtype != FLOW_DT:
+        raise ValueError("flow must be a FLOW_DT array")
     _check(_lib.cfdx_write_restart(directory.encode(), slot, flow.shape[0], _ptr(flow), iteration, digits))
 
 

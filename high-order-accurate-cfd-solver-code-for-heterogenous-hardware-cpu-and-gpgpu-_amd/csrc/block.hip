@@ -315,9 +315,9 @@ int upload_batch(cfdx_batch *q)
 // ---------------------------------------------------------------------------
 extern "C" int64_t cfdx_block_bytes(const cfdx_block_desc *d)
 {
-    if (!d || d->ni < 1 || d->nj < 1 || d->nk < 1) return -1;
-    if (d->precision != CFDX_FP64 && d->precision != CFDX_MIXED) return -1;
-    if (d->periodic < 0 || d->periodic > 7) return -1;
+    if (!d) return cfdx::fail(CFDX_E_INVALID, "block_bytes: null descriptor");
+    const int rc = check_desc(d);
+    if (rc != CFDX_OK) return rc;
     return bytes_of(d);
 }
 

@@ -173,7 +173,8 @@ typedef struct {
  * padded (i + off[0], j + off[1], k + off[2]). */
 int cfdx_block_padded(const cfdx_block_desc *desc, int32_t dims[3], int32_t off[3]);
 
-/* Bytes of device memory a block of this shape holds. */
+/* Bytes of device memory a block of this shape holds; a negative CFDX_E_* code (message in
+ * cfdx_last_error) for a descriptor cfdx_block_create would refuse. */
 int64_t cfdx_block_bytes(const cfdx_block_desc *desc);
 int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfdx_block **out);
 int cfdx_block_destroy(cfdx_block *b);
