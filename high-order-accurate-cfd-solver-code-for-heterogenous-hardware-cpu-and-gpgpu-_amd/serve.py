@@ -124,11 +124,19 @@ def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) ->
 
 
 class Server:
+    """Continuous batching on one GPU. The batched steps run on the main context's stream; blocks
+    are created, set up and read back on a second context (its own stream) while a chunk of
+    steps runs, so block set-up, result copies and host work overlap the device work (pipelined:
+    one chunk in flight). Memory of retired blocks is released after the next chunk completes."""
+
     def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
                  budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
                  grid_cap: int = 0, batched: bool = True, keep_flow: bool = True, prep_workers: int = 8,
-                 max_nodes: int | None = None):
+                 max_nodes: int | None = None, pipelined: bool = True):
         self.ctx = ctx or Ctx()
+        # block set-up and result copies (same device, another stream)
+        self.io = Ctx(self.ctx.device) if pipelined else self.ctx
+        self.pipelined = pipelined
         self.precision = precision
         self.reserve = reserve_bytes
         self.budget = budget_bytes
@@ -161,77 +169,105 @@ class Server:
             raise ValueError(clock)
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         queue: collections.deque[Request] = collections.deque()
-        active: dict[int, dict] = {}
+        active: dict[int, dict] = {}      # in the batch
+        admitted: list[dict] = []          # set up, joining the batch at the next chunk boundary
         results: dict[int, Result] = {}
+        retired: list = []                 # blocks whose memory is released after the next sync
         used = 0
-        vclock = 0  # steps executed (steps clock)
+        vclock = 0                         # steps executed (steps clock)
+        inflight = 0                       # steps of the chunk running on the device (0: none)
         t0 = time.perf_counter()
 
         def now() -> float:
             return float(vclock) if clock == "steps" else time.perf_counter() - t0
 
+        def nodes_in_flight() -> int:
+            return sum(a["req"].nodes for a in active.values()) + sum(a["req"].nodes for a in admitted)
+
         preps: dict[int, concurrent.futures.Future] = {}
-        while pending or queue or active:
+        while pending or queue or active or admitted or inflight:
             t = now()
             while pending and pending[0].arrival <= t:
                 req = pending.popleft()
                 queue.append(req)
                 if req.grid == "cgrid":
                     preps[req.rid] = self.pool.submit(prepare, req)
-            # admission: FIFO while memory and slots allow, once the host set-up is ready (it is
-            # waited for when nothing else could run)
-            while queue and len(active) < self.max_active and self._fits(
-                    queue[0], used, sum(a["req"].nodes for a in active.values()), bool(active)):
+            # admission: FIFO while memory, slots and the node cap allow, once the host set-up is
+            # ready (waited for when nothing else could run); the block is built on the set-up
+            # stream while a chunk may be running
+            busy = bool(active or admitted)
+            while queue and len(active) + len(admitted) < self.max_active and self._fits(
+                    queue[0], used, nodes_in_flight(), busy):
                 req = queue[0]
                 fut = preps.get(req.rid)
-                if fut is not None and not fut.done() and active:
+                if fut is not None and not fut.done() and (busy or inflight):
                     break
                 queue.popleft()
                 prep = preps.pop(req.rid).result() if fut is not None else None
-                b, dt = make_block(req, self.precision, self.ctx, prep)
-                self.batch.add(b)
+                b, dt = make_block(req, self.precision, self.io, prep)
                 nb = req.bytes(self.precision)
                 used += nb
-                active[req.rid] = dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb)
-            if queue and not active:
+                admitted.append(dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb))
+                busy = True
+            if queue and not (active or admitted or inflight):
                 raise RuntimeError(f"request {queue[0].rid} ({queue[0].ni}x{queue[0].nj}x{queue[0].nk}) does not fit "
                                    "in the free device memory")
-            if not active:
-                if pending:  # idle until the next arrival
-                    if clock == "steps":
-                        vclock = max(vclock, int(math.ceil(pending[0].arrival)))
-                    else:
-                        time.sleep(max(0.0, pending[0].arrival - now()))
-                continue
-            self.peak_active = max(self.peak_active, len(active))
-            self.peak_bytes = max(self.peak_bytes, used)
-            k = min(self.chunk, min(a["left"] for a in active.values()))
-            if clock == "steps" and pending:
-                k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
-            self.batch.step(k, self.graph)
-            self.ctx.sync()
-            self.chunks += 1
-            vclock += k
+            # the chunk in flight completes
             done = []
-            for rid, a in active.items():
-                a["left"] -= k
-                if a["left"] == 0:
-                    done.append(rid)
-            for rid in done:
-                a = active.pop(rid)
+            if inflight:
+                self.ctx.sync()
+                self.chunks += 1
+                vclock += inflight
+                for rid, a in active.items():
+                    a["left"] -= inflight
+                    if a["left"] == 0:
+                        done.append(active[rid])
+                inflight = 0
+                for b in retired:
+                    b.close()
+                retired = []
+            for a in done:
+                active.pop(a["req"].rid)
+                self.batch.remove(a["block"])
+            for a in admitted:
+                self.batch.add(a["block"])
+                active[a["req"].rid] = a
+            admitted = []
+            # next chunk, asynchronous
+            if active:
+                self.peak_active = max(self.peak_active, len(active))
+                self.peak_bytes = max(self.peak_bytes, used)
+                k = min(self.chunk, min(a["left"] for a in active.values()))
+                if clock == "steps" and pending:
+                    k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
+                self.batch.step(k, self.graph)
+                inflight = k
+                if not self.pipelined:
+                    self.ctx.sync()
+            # results of the requests that finished, copied on the set-up stream during the chunk
+            for a in done:
                 b = a["block"]
                 flow = b.get_flow() if self.keep_flow else None
-                if not self.keep_flow:
-                    self.ctx.sync()
-                self.batch.remove(b)
-                b.close()
                 used -= a["bytes"]
-                results[rid] = Result(rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow)
+                results[a["req"].rid] = Result(a["req"].rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow)
+                retired.append(b)
+            if not (active or admitted or inflight) and pending and not queue:
+                for b in retired:
+                    b.close()
+                retired = []
+                if clock == "steps":   # idle until the next arrival
+                    vclock = max(vclock, int(math.ceil(pending[0].arrival)))
+                else:
+                    time.sleep(max(0.0, pending[0].arrival - now()))
+        for b in retired:
+            b.close()
         return results
 
     def close(self) -> None:
         self.batch.close()
         self.pool.shutdown(wait=True)
+        if self.io is not self.ctx:
+            self.io.close()
 
 
 def report(requests: list[Request], results: dict[int, Result]) -> dict:
