@@ -502,26 +502,15 @@ extern "C" int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow)
     if (!b || !flow) return cfdx::fail(CFDX_E_INVALID, "get_flow: null argument");
     CFDX_TRY(cfdx::enter(b->ctx));
     const long long n = (long long)b->desc.ni * b->desc.nj * b->desc.nk;
-    std::vector<double> h((size_t)6 * n);
-    // the face-flux arrays are scratch between stages: 15 n flux-type values >= 5 n doubles
-    double *tmp = (double *)b->B.FH[0][0];
-    cfdx::blk::prims(b->B, b->desc.precision, b->ctx->stream, tmp);
+    // the viscous node-flux and face-flux arrays (contiguous, 27 flux-type values per stored node
+    // >= 72 bytes + 16) are scratch between steps: the records are built there on the device,
+    // from a 16-byte aligned start
+    cfdx_flow *tmp = (cfdx_flow *)(((uintptr_t)b->B.Fv[0][0] + 15) & ~(uintptr_t)15);
+    cfdx::blk::flow_out(b->B, b->desc.precision, b->ctx->stream, tmp, b->last_mach);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), tmp, (size_t)5 * n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
-    std::vector<double> mu(b->B.n);
-    if (e == hipSuccess) e = hipMemcpyAsync(mu.data(), b->B.mu, (size_t)b->B.n * 8, hipMemcpyDeviceToHost, b->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(flow, tmp, (size_t)n * sizeof(cfdx_flow), hipMemcpyDeviceToHost, b->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
     if (e != hipSuccess) return cfdx::hip_fail(e, "get_flow");
-    gather_owned(b, mu.data(), h.data() + 5 * n);
-    const double M = b->last_mach;
-    for (long long g = 0; g < n; ++g) {
-        cfdx_flow &f = flow[g];
-        f.rho = h[g]; f.u = h[n + g]; f.v = h[2 * n + g]; f.w = h[3 * n + g]; f.p = h[4 * n + g]; f.mu = h[5 * n + g];
-        // Weno.cl:3132-3135
-        f.e = f.p / (0.4 * f.rho);
-        f.t = (1.4 * M * M) * (f.p / f.rho);
-        f.a = std::sqrt(1.4 * f.p / f.rho);
-    }
     return CFDX_OK;
 }
 

@@ -93,6 +93,8 @@ void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool an
 // P and U0 of the owned nodes from double primitives staged component-major in owned order
 void base(const BlockDev &B, int precision, hipStream_t st, const double *stage);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
+// FLOW records (cfdx_flow) of the owned nodes, owned order, into device memory
+void flow_out(const BlockDev &B, int precision, hipStream_t st, cfdx_flow *out, double mach);
 uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg);
 // side conditions of the bounded blocks of a list (mode 0..2 after that RK stage, mode 2 also
 // refreshing U0), side by side in the order i-, i+, j-, j+, k-, k+; stage() runs it after the update

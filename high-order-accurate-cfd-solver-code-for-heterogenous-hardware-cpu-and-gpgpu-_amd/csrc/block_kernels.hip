@@ -816,6 +816,42 @@ void boundary(const LaunchList &L, int precision, int mode, hipStream_t st)
     else launch_bc<double>(L, mode, st);
 }
 
+// get_flow: the reference's FLOW record of every owned node, owned order (Weno.cl:3126-3135 for
+// t, a, e; mu of the last viscous pass). The primitives are those of the last stage (plain form)
+// or recovered in double from the RK base (mixed form).
+__global__ void k_flow_out(BlockDev B, cfdx_flow *__restrict__ out, double mach, int from_base)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x, no = owned_count(B);
+    if (o >= no) return;
+    const uint32_t g = owned_node(B, o);
+    double rho, u, v, w, p;
+    if (from_base) {
+        const double idet = 1.0 / ld(B.det, g);
+        rho = idet * ld(B.U0[0], g);
+        const double ir = 1.0 / rho;
+        u = (idet * ld(B.U0[1], g)) * ir; v = (idet * ld(B.U0[2], g)) * ir; w = (idet * ld(B.U0[3], g)) * ir;
+        const double e = (idet * ld(B.U0[4], g)) * ir - 0.5 * (u * u + v * v + w * w);
+        p = e * rho * 0.4;
+    } else {
+        rho = ld((const double *)B.P[0], g); u = ld((const double *)B.P[1], g); v = ld((const double *)B.P[2], g);
+        w = ld((const double *)B.P[3], g); p = ld((const double *)B.P[4], g);
+    }
+    // the record's 9 doubles in order u, v, w, p, t, rho, mu, a, e (out is 8-byte aligned)
+    double *r = (double *)out + 9 * (size_t)o;
+    r[0] = u; r[1] = v; r[2] = w; r[3] = p;
+    r[4] = (1.4 * mach * mach) * (p / rho);
+    r[5] = rho;
+    r[6] = ld(B.mu, g);
+    r[7] = sqrt(1.4 * p / rho);
+    r[8] = p / (0.4 * rho);
+}
+
+void flow_out(const BlockDev &B, int precision, hipStream_t st, cfdx_flow *out, double mach)
+{
+    const unsigned nb = (unsigned)(((uint32_t)B.ni * B.nj * B.nk + 255) / 256);
+    hipLaunchKernelGGL(k_flow_out, dim3(nb), dim3(256), 0, st, B, out, mach, precision == CFDX_MIXED ? 1 : 0);
+}
+
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
 {
     const unsigned nb = (unsigned)(((uint32_t)B.ni * B.nj * B.nk + 255) / 256);
