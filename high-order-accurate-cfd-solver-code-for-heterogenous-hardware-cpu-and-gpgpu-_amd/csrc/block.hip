@@ -467,10 +467,16 @@ extern "C" int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo)
 {
     if (!b || !geo) return cfdx::fail(CFDX_E_INVALID, "set_geometry_soa: null argument");
     CFDX_TRY(cfdx::enter(b->ctx));
-    for (size_t q = 0; q < (size_t)22 * b->B.n; ++q)
-        if (!std::isfinite(geo[q])) return cfdx::fail(CFDX_E_NONFINITE, "set_geometry_soa: value %zu is not finite", q);
     CFDX_HIP(hipMemcpyAsync((void *)b->B.det, geo, (size_t)22 * b->B.n * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry_soa copy");
+    // finiteness checked on the device; the counter lives in the face-flux scratch
+    unsigned *cnt = (unsigned *)b->B.FH[0][0];
+    CFDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), b->ctx->stream), "set_geometry_soa clear");
+    cfdx::blk::count_nonfinite(b->B.det, (size_t)22 * b->B.n, cnt, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("k_nonfinite"));
+    unsigned bad = 0;
+    CFDX_HIP(hipMemcpyAsync(&bad, cnt, sizeof bad, hipMemcpyDeviceToHost, b->ctx->stream), "set_geometry_soa check");
     CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "set_geometry_soa sync");
+    if (bad) return cfdx::fail(CFDX_E_NONFINITE, "set_geometry_soa: the geometry holds values that are not finite");
     return CFDX_OK;
 }
 

@@ -93,6 +93,8 @@ void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool an
 // P and U0 of the owned nodes from double primitives staged component-major in owned order
 void base(const BlockDev &B, int precision, hipStream_t st, const double *stage);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);
+// *count += the number of workgroups that found a value that is not finite in p[0..n)
+void count_nonfinite(const double *p, size_t n, unsigned *count, hipStream_t st);
 // FLOW records (cfdx_flow) of the owned nodes, owned order, into device memory
 void flow_out(const BlockDev &B, int precision, hipStream_t st, cfdx_flow *out, double mach);
 uint32_t tiles_of(const BlockDev &B, int kind, const TileCfg &cfg);

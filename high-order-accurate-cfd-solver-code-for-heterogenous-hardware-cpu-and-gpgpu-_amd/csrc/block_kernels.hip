@@ -852,6 +852,21 @@ void flow_out(const BlockDev &B, int precision, hipStream_t st, cfdx_flow *out, 
     hipLaunchKernelGGL(k_flow_out, dim3(nb), dim3(256), 0, st, B, out, mach, precision == CFDX_MIXED ? 1 : 0);
 }
 
+// number of workgroups that met a value that is not finite among p[0..n)
+__global__ void __launch_bounds__(256) k_nonfinite(const double *__restrict__ p, size_t n, unsigned *__restrict__ count)
+{
+    bool bad = false;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        bad |= !__builtin_isfinite(p[q]);
+    if (__syncthreads_or(bad) && threadIdx.x == 0) atomicAdd(count, 1u);
+}
+
+void count_nonfinite(const double *p, size_t n, unsigned *count, hipStream_t st)
+{
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_nonfinite, dim3(nb), dim3(256), 0, st, p, n, count);
+}
+
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
 {
     const unsigned nb = (unsigned)(((uint32_t)B.ni * B.nj * B.nk + 255) / 256);

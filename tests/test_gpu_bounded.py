@@ -142,3 +142,15 @@ def test_batch_with_bounded_blocks(ctx):
         for f in FIELDS:
             assert np.array_equal(got[f], want[f]), f
     q.close()
+
+
+def test_geometry_not_finite_refused(ctx):
+    d, xyz, geo, mach, aoa, pk = box_case(4, (3, 2, 1, 1, 0, 0), (14, 12, 6))
+    bad = geo.copy()
+    bad[7 * (geo.size // 22) + 123] = np.nan      # one metric value of one node
+    b = cfdx.Block(d.ni, d.nj, d.nk, ctx=ctx, desc=cfdx.BlockDesc.from_buffer_copy(d))
+    with pytest.raises(cfdx.CfdxError) as e:
+        b.set_geometry_soa(bad)
+    assert e.value.code == cfdx.E_NONFINITE
+    b.set_geometry_soa(geo)                       # a finite geometry is taken afterwards
+    b.close()
