@@ -102,6 +102,7 @@ EXPORTS = {
     "cfdx_sync": (C.c_int, [_vp]),
     "cfdx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "cfdx_mem_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "cfdx_pool_trim": (C.c_int, [_vp]),
     "cfdx_device_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "cfdx_device_free": (C.c_int, [_vp, _vp]),
     "cfdx_copy_to_device": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
@@ -402,6 +403,10 @@ class Ctx:
         f, t = _i64(0), _i64(0)
         _check(_lib.cfdx_mem_info(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+    def pool_trim(self) -> None:
+        """Frees the block memory kept for reuse on this device (cfdx_pool_trim)."""
+        _check(_lib.cfdx_pool_trim(self.h))
 
     def op_count(self, params=None) -> dict:
         """Static operation count of the scheme's arithmetic per face / node (cfdx_op_count):

@@ -18,7 +18,8 @@ struct cfdx_block {
     cfdx_block_desc desc{};
     BlockDev B{};                 // host image of the device descriptor
     void *mem = nullptr;          // one allocation for every array of the block
-    size_t bytes = 0;
+    size_t bytes = 0;             // the arrays (bytes_of)
+    size_t alloc = 0;             // the allocation: arrays, then dB and dprefix
     BlockDev *dB = nullptr;       // device copy of B (single-block launches)
     uint32_t *dprefix = nullptr;  // 4 x 2 tile offsets of the single-block launch list
     uint64_t version = 1;         // bumped whenever B changes
@@ -72,6 +73,9 @@ int64_t stored_nodes(const cfdx_block_desc *d)
     padded_dims(d, dims, off);
     return (int64_t)dims[0] * dims[1] * dims[2];
 }
+
+// the block descriptor and the single-block prefix table after the arrays
+size_t tail_bytes() { return ((sizeof(BlockDev) + 255) / 256) * 256 + 256; }
 
 int64_t bytes_of(const cfdx_block_desc *d)
 {
@@ -318,7 +322,7 @@ extern "C" int64_t cfdx_block_bytes(const cfdx_block_desc *d)
     if (!d) return cfdx::fail(CFDX_E_INVALID, "block_bytes: null descriptor");
     const int rc = check_desc(d);
     if (rc != CFDX_OK) return rc;
-    return bytes_of(d);
+    return bytes_of(d) + (int64_t)tail_bytes();
 }
 
 extern "C" int cfdx_block_padded(const cfdx_block_desc *desc, int32_t dims[3], int32_t off[3])
@@ -342,14 +346,15 @@ extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfd
     cfdx::ctx_retain(ctx);
     b->desc = *desc;
     b->bytes = (size_t)bytes_of(desc);
-    hipError_t e = hipMalloc(&b->mem, b->bytes);
-    if (e != hipSuccess) {
+    b->mem = cfdx::pool_take(ctx->device, b->bytes + tail_bytes(), &b->alloc);
+    if (!b->mem) {
+        cfdx::ctx_release(ctx);
         delete b;
-        return cfdx::fail(CFDX_E_NOMEM, "block_create: hipMalloc %zu bytes: %s", (size_t)bytes_of(desc), hipGetErrorString(e));
+        return cfdx::fail(CFDX_E_NOMEM, "block_create: no device memory for %zu bytes", b->bytes + tail_bytes());
     }
-    e = hipMalloc((void **)&b->dB, sizeof(BlockDev));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->dprefix, 8 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemsetAsync(b->mem, 0, b->bytes, ctx->stream);
+    b->dB = (BlockDev *)((char *)b->mem + b->bytes);
+    b->dprefix = (uint32_t *)((char *)b->dB + ((sizeof(BlockDev) + 255) / 256) * 256);
+    hipError_t e = hipMemsetAsync(b->mem, 0, b->bytes, ctx->stream);
     for (auto &ev : b->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -410,9 +415,7 @@ extern "C" int cfdx_block_destroy(cfdx_block *b)
     (void)hipStreamSynchronize(b->ctx->stream);
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
-    if (b->mem) (void)hipFree(b->mem);
-    if (b->dB) (void)hipFree(b->dB);
-    if (b->dprefix) (void)hipFree(b->dprefix);
+    if (b->mem) cfdx::pool_give(b->ctx->device, b->mem, b->alloc);   // dB and dprefix live at its end
     if (b->mon) (void)hipFree(b->mon);
     cfdx_ctx *ctx = b->ctx;
     delete b;

@@ -1,5 +1,99 @@
-// Context lifecycle: one device, one stream (caller's or owned).
+// Context lifecycle: one device, one stream (caller's or owned); the per-device pool of block
+// memory.
 #include "ctx.hpp"
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace cfdx {
+namespace {
+// Block allocations returned by cfdx_block_destroy are kept for reuse instead of freed:
+// hipFree waits for the whole device, and a serving loop creates and retires blocks of a few
+// recurring sizes while other work runs. Bounded by a quarter of the device memory.
+struct Pool {
+    std::multimap<size_t, void *> free;   // size -> allocation
+    size_t cached = 0, cap = 0;
+};
+std::mutex pool_mutex;
+std::vector<Pool> pools;
+
+Pool &pool_of(int device)
+{
+    if ((int)pools.size() <= device) pools.resize(device + 1);
+    Pool &p = pools[device];
+    if (!p.cap) {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) == hipSuccess) p.cap = t / 4;
+    }
+    return p;
+}
+}  // namespace
+
+void *pool_take(int device, size_t bytes, size_t *got)
+{
+    {
+        std::lock_guard<std::mutex> lock(pool_mutex);
+        Pool &p = pool_of(device);
+        auto it = p.free.lower_bound(bytes);
+        if (it != p.free.end() && it->first <= bytes + bytes / 4) {
+            void *ptr = it->second;
+            *got = it->first;
+            p.cached -= it->first;
+            p.free.erase(it);
+            return ptr;
+        }
+    }
+    void *ptr = nullptr;
+    if (hipMalloc(&ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim(device);
+        if (hipMalloc(&ptr, bytes) != hipSuccess) return nullptr;
+    }
+    *got = bytes;
+    return ptr;
+}
+
+void pool_give(int device, void *ptr, size_t bytes)
+{
+    if (!ptr) return;
+    {
+        std::lock_guard<std::mutex> lock(pool_mutex);
+        Pool &p = pool_of(device);
+        if (p.cached + bytes <= p.cap) {
+            p.free.emplace(bytes, ptr);
+            p.cached += bytes;
+            return;
+        }
+    }
+    (void)hipFree(ptr);
+}
+
+void pool_trim(int device)
+{
+    std::multimap<size_t, void *> out;
+    {
+        std::lock_guard<std::mutex> lock(pool_mutex);
+        Pool &p = pool_of(device);
+        out.swap(p.free);
+        p.cached = 0;
+    }
+    for (auto &kv : out) (void)hipFree(kv.second);
+}
+
+size_t pool_cached(int device)
+{
+    std::lock_guard<std::mutex> lock(pool_mutex);
+    return pool_of(device).cached;
+}
+}  // namespace cfdx
+
+extern "C" int cfdx_pool_trim(cfdx_ctx *ctx)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    cfdx::pool_trim(ctx->device);
+    return CFDX_OK;
+}
 
 extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
 {
@@ -100,7 +194,7 @@ extern "C" int cfdx_mem_info(cfdx_ctx *ctx, int64_t *free_bytes, int64_t *total_
     CFDX_TRY(cfdx::enter(ctx));
     size_t f = 0, t = 0;
     CFDX_HIP(hipMemGetInfo(&f, &t), "hipMemGetInfo");
-    *free_bytes = (int64_t)f;
+    *free_bytes = (int64_t)(f + cfdx::pool_cached(ctx->device));   // pooled block memory is available
     *total_bytes = (int64_t)t;
     return CFDX_OK;
 }

@@ -19,6 +19,14 @@ struct cfdx_ctx {
 namespace cfdx {
 void ctx_retain(cfdx_ctx *ctx);
 int ctx_release(cfdx_ctx *ctx);
+
+// Device memory for blocks: a cached allocation of at least `bytes` (at most a quarter more;
+// *got = its size) or a new one; nullptr when the device has none. pool_give keeps it for
+// reuse (up to a quarter of the device memory) or frees it; pool_trim frees what is kept.
+void *pool_take(int device, size_t bytes, size_t *got);
+void pool_give(int device, void *ptr, size_t bytes);
+void pool_trim(int device);
+size_t pool_cached(int device);
 }  // namespace cfdx
 
 namespace cfdx {

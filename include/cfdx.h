@@ -98,8 +98,13 @@ int cfdx_sync(cfdx_ctx *ctx);
 /* Number of HIP devices visible to this process. */
 int cfdx_device_count(int *count);
 
-/* Free and total device memory of the context's device (hipMemGetInfo); used to admit work. */
+/* Free and total device memory of the context's device (hipMemGetInfo; free includes the block
+ * memory pooled for reuse); used to admit work. */
 int cfdx_mem_info(cfdx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+/* Frees the block memory the library keeps for reuse on ctx's device (cfdx_block_destroy returns
+ * a block's allocation to a per-device pool, bounded by a quarter of the device memory, instead
+ * of freeing it: freeing waits for the whole device). cfdx_mem_info counts pooled memory as free. */
+int cfdx_pool_trim(cfdx_ctx *ctx);
 
 /* Device memory and copies on the context's device (the drop-in kernels take
  * device pointers; callers may equally use their own hipMalloc'd buffers).

@@ -131,3 +131,33 @@ def test_serving_airfoils_alone_vs_mix(ctx):
         for f in FIELDS:
             assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
             assert np.isfinite(res[r.rid].flow[f]).all()
+
+
+def test_block_memory_pool(ctx):
+    """Destroyed blocks return their memory to a per-device pool (no hipFree, which waits for the
+    device); mem_info counts it as free, a new block of the same size reuses it, zeroed and
+    giving the same results, and pool_trim frees it."""
+    ctx.pool_trim()
+    f0, _ = ctx.mem_info()
+    need = cfdx.Block.bytes_for(64, 64, 64, cfdx.FP64)
+
+    def run():
+        b = cfdx.Block(64, 64, 64, ctx=ctx)
+        b.init_case(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.01, phases=(0.1, 0.2, 0.3), init="tgv", mach=0.2)
+        prm = cfdx.default_params(mach=0.2, reyl=800.0)
+        b.step(2, 0.2, b.cfl_dt(0.5), prm)
+        return b, b.get_flow()
+
+    b1, r1 = run()
+    f1, _ = ctx.mem_info()
+    assert f0 - f1 >= need
+    b1.close()
+    f2, _ = ctx.mem_info()
+    assert abs(f2 - f0) <= 64 << 20
+    b2, r2 = run()
+    for f in FIELDS:
+        assert np.array_equal(r1[f], r2[f])
+    b2.close()
+    ctx.pool_trim()
+    f3, _ = ctx.mem_info()
+    assert abs(f3 - f0) <= 64 << 20
