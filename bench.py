@@ -7,14 +7,15 @@ viscous/SGS fluxes) and RK3. A step = one full SSP-RK3 step over the whole grid 
 the state resident in HBM. Prints one JSON line.
 
 Beside the headline the same JSON line carries (unless --legs drops them): the other precision
-on the same case with per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the 64-cube batched
+on the same case with per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the same flow at 256^3
+in FP64 (one GPU's share of a 512^3 run, "one_gpu_share_256"), the 64-cube batched
 mixed-precision graph step ("batched_c4"), the NACA 0012 C-grid in mixed and plain FP64 with
 per-field errors ("airfoil_c3"), and a bounded sample of the Poisson stream of NACA C-grid solves
 served by continuous batching vs plain FP64 one at a time ("serving": p50 / p99 latency,
 throughput).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1|c3] [--precision fp64|mixed]
-                    [--legs compare,c4,c3,serve|none] [--no-cpu] [--no-tune]
+                    [--legs compare,ns256,c4,c3,serve|none] [--no-cpu] [--no-tune]
 
 --workload c3 makes the NACA 0012 C-grid block (384x128x32, free stream start) the timed block.
 """
@@ -236,6 +237,33 @@ def field_errors(a, b):
     return out
 
 
+def one_gpu_share(cfdx, tune: bool, steps: int = 10, warmup: int = 2):
+    """The C2 map and flow at 256^3, plain FP64: one GPU's share of a 512^3 run split 2x2x2 (the
+    periodic wrap stands in for the neighbours' halo), set up on the device."""
+    spec, cdef, pkw, _ = workload("ns256")
+    prm = cfdx.default_params(**pkw)
+    mach = cdef["kw"]["mach"]
+    tiles = tuned_tiles(cfdx, cdef, cfdx.FP64, mach, prm) if tune else None
+    b, dt = make_block(cfdx, cdef, cfdx.FP64, prm)
+    if tiles:
+        b.set_tiles(tuple(tiles["tiles"]))
+    ctx = b.ctx
+    b.step(warmup, mach, dt, prm)
+    ctx.sync()
+    ctx.record(0)
+    b.step(steps, mach, dt, prm)
+    ctx.record(1)
+    ctx.sync()
+    ms = ctx.elapsed_ms(0, 1)
+    import numpy as np
+    finite = bool(np.isfinite(cfdx_matrix(b.get_flow())).all())
+    b.close()
+    n = cdef["ni"] * cdef["nj"] * cdef["nk"]
+    return {"workload": spec["workload"], "precision": "fp64", "steps": steps, "value": n * steps / (ms / 1e3),
+            "unit": "cell-updates/s", "ms_per_step": ms / steps, "state_finite": finite,
+            "tiles": tiles["tiles"] if tiles else None}
+
+
 C4_EDGES = (32, 48, 64, 96, 128)
 
 
@@ -401,9 +429,9 @@ def main() -> int:
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--legs", default="compare,c4,c3,serve",
-                    help="extra legs after the headline: compare (other precision, per-field errors), c4, c3, serve "
-                         "(none: the headline only)")
+    ap.add_argument("--legs", default="compare,ns256,c4,c3,serve",
+                    help="extra legs after the headline: compare (other precision, per-field errors), ns256 (256^3 "
+                         "FP64, one GPU's share of 512^3), c4, c3, serve (none: the headline only)")
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--serve-kind", default="airfoil", choices=["airfoil", "periodic"])
     ap.add_argument("--c4-steps", type=int, default=5)
@@ -537,6 +565,9 @@ def main() -> int:
     del case
     block.close()
 
+    if "ns256" in legs:
+        line["one_gpu_share_256"] = one_gpu_share(cfdx, args.tune)
+        print(f"bench: 256^3 {line['one_gpu_share_256']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
     if "c4" in legs:
         line["batched_c4"] = batched_c4(cfdx, args.c4_steps, 1)
         print(f"bench: c4 {line['batched_c4']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
