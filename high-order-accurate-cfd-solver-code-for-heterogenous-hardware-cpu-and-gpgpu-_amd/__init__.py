@@ -146,6 +146,7 @@ EXPORTS = {
     "cfdx_batch_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_op_count": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(_i64)]),
     "cfdx_block_set_monitor": (C.c_int, [_vp, C.c_int]),
+    "cfdx_block_set_check": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_residual": (C.c_int, [_vp, C.POINTER(_d)]),
     "cfdx_write_restart": (C.c_int, [C.c_char_p, _i32, _i64, _vp, _i64, _i32]),
     "cfdx_restart_latest": (C.c_int, [C.c_char_p, _i64, C.POINTER(_i32), C.POINTER(_i64)]),
@@ -621,6 +622,10 @@ class Block:
 
     def set_timing(self, on: bool) -> None:
         _check(_lib.cfdx_block_set_timing(self.h, int(on)))
+
+    def set_check(self, on: bool = True) -> None:
+        """Non-finite check after every step() call (CfdxError E_NONFINITE when the state breaks)."""
+        _check(_lib.cfdx_block_set_check(self.h, int(on)))
 
     def set_monitor(self, on: bool = True) -> None:
         _check(_lib.cfdx_block_set_monitor(self.h, int(on)))

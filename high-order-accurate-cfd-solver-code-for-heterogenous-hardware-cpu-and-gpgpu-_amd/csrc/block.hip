@@ -25,6 +25,8 @@ struct cfdx_block {
     uint64_t version = 1;         // bumped whenever B changes
     uint64_t uploaded = 0;        // version of B in dB
     bool configured = false;
+    bool check = false;           // non-finite check of the state after each cfdx_block_step
+    long long steps_done = 0;     // steps taken by cfdx_block_step (for the check's message)
     double last_mach = 0.5;
     bool timing = false;
     hipEvent_t ev[6] = {};
@@ -580,6 +582,26 @@ extern "C" int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nst
     b->launches = 0;
     for (int s = 0; s < nsteps; ++s)
         for (int j = 0; j < 3; ++j) CFDX_TRY(run_stage(b, j));
+    b->steps_done += nsteps;
+    if (b->check && nsteps > 0) {
+        // the RK base holds the state after the step; the counter lives in the face-flux scratch
+        unsigned *cnt = (unsigned *)b->B.FH[0][0];
+        CFDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), b->ctx->stream), "block_step check clear");
+        cfdx::blk::count_nonfinite(b->B.U0[0], (size_t)5 * b->B.n, cnt, b->ctx->stream);
+        CFDX_TRY(cfdx::check_launch("k_nonfinite"));
+        unsigned bad = 0;
+        CFDX_HIP(hipMemcpyAsync(&bad, cnt, sizeof bad, hipMemcpyDeviceToHost, b->ctx->stream), "block_step check");
+        CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "block_step check sync");
+        if (bad)
+            return cfdx::fail(CFDX_E_NONFINITE, "block_step: the state is not finite after step %lld", b->steps_done);
+    }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_set_check(cfdx_block *b, int on)
+{
+    if (!b) return cfdx::fail(CFDX_E_INVALID, "set_check: null block");
+    b->check = on != 0;
     return CFDX_OK;
 }
 

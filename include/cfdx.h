@@ -240,6 +240,10 @@ int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face, double *m
  * those of the last step (out[0..2], synchronous). set_monitor(on) allocates the partial
  * maxima on first use. */
 int cfdx_block_set_monitor(cfdx_block *b, int on);
+/* Non-finite detector (no counterpart in the reference, whose runs carry NaN on): with it on,
+ * every cfdx_block_step ends by checking the block's state on the device and returns
+ * CFDX_E_NONFINITE if any value is NaN or infinite (one device pass over U0 and a sync). */
+int cfdx_block_set_check(cfdx_block *b, int on);
 int cfdx_block_residual(cfdx_block *b, double out[3]);
 
 /* Per-kernel device time of the timed steps: ms[0] viscous, ms[1..3] face sweeps i, j, k,

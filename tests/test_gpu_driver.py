@@ -106,3 +106,20 @@ def test_driver_restart_resume(ctx, tmp_path):
     for f in FIELDS:
         assert np.array_equal(got[f], want[f]), f
     assert (tmp_path / "a" / "nodefile_3.dat").exists()
+
+
+def test_nonfinite_check(ctx):
+    """With the check on, a step whose state turns NaN (a zero density in the initial state) returns
+    CFDX_E_NONFINITE; a healthy block steps on."""
+    prm = cfdx.default_params(mach=0.3, reyl=800.0)
+    b = tgv(ctx)
+    dt = b.cfl_dt(0.5)
+    b.set_check(True)
+    b.step(2, 0.3, dt, prm)
+    flow = b.get_flow()
+    flow["rho"][100] = 0.0
+    b.set_flow(flow)
+    with pytest.raises(cfdx.CfdxError) as e:
+        b.step(1, 0.3, dt, prm)
+    assert e.value.code == cfdx.E_NONFINITE and "after step 3" in str(e.value)
+    b.close()
