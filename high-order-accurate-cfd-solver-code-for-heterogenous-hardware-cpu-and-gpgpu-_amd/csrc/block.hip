@@ -83,9 +83,11 @@ int64_t bytes_of(const cfdx_block_desc *d)
 {
     const int64_t n = stored_nodes(d);
     // 33 double arrays (geometry 22, U0 5, Us 5, mu 1) + 32 flux-type arrays (P 5, Fv 12, FH 15)
-    // + one byte of loc code per node for a bounded block
+    // + in a mixed block float copies of the 22 geometry arrays + one byte of loc code per node
+    // for a bounded block
     const int64_t codes = d->periodic == 7 ? 0 : ((n + 255) / 256) * 256;
-    return n * (33 * 8 + 32 * (int64_t)flux_size(d->precision)) + codes + 256;
+    const int64_t geo32 = d->precision == CFDX_MIXED ? 22 * 4 : 0;
+    return n * (33 * 8 + 32 * (int64_t)flux_size(d->precision) + geo32) + codes + 256;
 }
 
 int check_desc(const cfdx_block_desc *d)
@@ -390,6 +392,11 @@ extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfd
     for (auto &a : B.P) a = takeS();
     for (auto &r : B.Fv) for (auto &a : r) a = takeS();
     for (auto &r : B.FH) for (auto &a : r) a = takeS();
+    B.geo32 = nullptr;
+    if (desc->precision == CFDX_MIXED) {
+        B.geo32 = (const float *)p;
+        p += (size_t)22 * n * sizeof(float);
+    }
     B.Lout = nullptr;
     B.code = nullptr;
     if (desc->periodic != 7) {
@@ -464,6 +471,8 @@ extern "C" int cfdx_block_set_geometry(cfdx_block *b, const double *det, const c
         for (int d = 0; d < 3; ++d) h[(19 + d) * n + g] = dd[2 * d];
     }
     CFDX_HIP(hipMemcpyAsync((void *)b->B.det, h.data(), h.size() * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry copy");
+    cfdx::blk::geometry_f32(b->B, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("k_geo32"));
     CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "set_geometry sync");
     return CFDX_OK;
 }
@@ -473,6 +482,8 @@ extern "C" int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo)
     if (!b || !geo) return cfdx::fail(CFDX_E_INVALID, "set_geometry_soa: null argument");
     CFDX_TRY(cfdx::enter(b->ctx));
     CFDX_HIP(hipMemcpyAsync((void *)b->B.det, geo, (size_t)22 * b->B.n * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry_soa copy");
+    cfdx::blk::geometry_f32(b->B, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("k_geo32"));
     // finiteness checked on the device; the counter lives in the face-flux scratch
     unsigned *cnt = (unsigned *)b->B.FH[0][0];
     CFDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), b->ctx->stream), "set_geometry_soa clear");
@@ -536,6 +547,8 @@ extern "C" int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c)
     CFDX_TRY(cfdx::enter(b->ctx));
     cfdx::blk::init_geometry(b->B, c->length, c->amp, c->phase, b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_geometry"));
+    cfdx::blk::geometry_f32(b->B, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("k_geo32"));
     cfdx::blk::init_state(b->B, c->length, c->amp, c->phase, c->init, c->mach, c->beta, c->center[0], c->center[1], b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_state"));
     cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream, b->B.Us[0]);

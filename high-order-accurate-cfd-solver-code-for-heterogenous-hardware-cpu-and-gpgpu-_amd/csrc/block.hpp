@@ -45,6 +45,8 @@ struct BlockDev {
     const double *m[9];         // node metrics zeta_x .. xi_z
     const double *kf[3][3];     // face metric row of direction d at the +1/2 face (zeta_?ip, eta_?jp, xi_?kp)
     const double *df[3];        // half-point det at ip, jp, kp
+    const float *geo32;         // mixed blocks: float copies of the 22 geometry arrays above, in their
+                                // order (det, m, kf, df); null in FP64 blocks
     void *P[5];                 // rho, u, v, w, p used by the flux kernels
     double *U0[5];              // RK base det*Q
     double *Us[5];              // stage vector U1 / U2
@@ -90,6 +92,8 @@ struct LaunchList {
 // ev (optional): 6 events recorded before the viscous kernel, after it, after each face sweep
 // and after the update
 void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev);
+// the float copies of the geometry of a mixed block (after every write of its double geometry)
+void geometry_f32(const BlockDev &B, hipStream_t st);
 // P and U0 of the owned nodes from double primitives staged component-major in owned order
 void base(const BlockDev &B, int precision, hipStream_t st, const double *stage);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);

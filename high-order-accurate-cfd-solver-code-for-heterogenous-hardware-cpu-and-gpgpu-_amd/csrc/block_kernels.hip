@@ -39,6 +39,17 @@ __device__ __forceinline__ void st(T *base, uint32_t i, T v)
     *(CFDX_GLOBAL T *)(p + i * (uint32_t)sizeof(T)) = v;
 }
 
+// Geometry array a (0 det, 1-9 node metrics, 10-18 face metric rows, 19-21 face dets) in the flux
+// type: the double arrays in an FP64 block, their float copies in a mixed block (the flux
+// arithmetic reads the metrics in S; the copies hold the same roundings a conversion at the load
+// would give, in half the bytes).
+template <typename S>
+__device__ __forceinline__ const S *geo(const BlockDev &B, int a);
+template <>
+__device__ __forceinline__ const double *geo<double>(const BlockDev &B, int a) { return B.det + (size_t)a * B.n; }
+template <>
+__device__ __forceinline__ const float *geo<float>(const BlockDev &B, int a) { return B.geo32 + (size_t)a * B.n; }
+
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // stored index of owned coordinates (i, j, k); a bounded direction may go 3 nodes beyond the
@@ -204,10 +215,10 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
 #pragma unroll
     for (int d = 0; d < 3; ++d) KSs[d] = S(KS[d]);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) m[c] = S(ld(B.m[c], g));
+    for (int c = 0; c < 9; ++c) m[c] = ld(geo<S>(B, 1 + c), g);
     S fv[3][5];
     const S mu = viscous_node(dUs, Uv, ld((const S *)B.P[1], g), ld((const S *)B.P[2], g), ld((const S *)B.P[3], g), m,
-                              S(ld(B.det, g)), KSs, B.vp, fv);
+                              ld(geo<S>(B, 0), g), KSs, B.vp, fv);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -308,8 +319,8 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
             const S rho = ld(P0, x), u = ld(P1, x), v = ld(P2, x), w = ld(P3, x), pr = ld(P4, x);
             S q[5], f[5];
             prim_to_q(rho, u, v, w, pr, q);
-            node_flux(rho, u, v, w, pr, S(ld(B.m[3 * DIR], x)), S(ld(B.m[3 * DIR + 1], x)), S(ld(B.m[3 * DIR + 2], x)),
-                      S(ld(B.det, x)), f);
+            node_flux(rho, u, v, w, pr, ld(geo<S>(B, 1 + 3 * DIR), x), ld(geo<S>(B, 2 + 3 * DIR), x),
+                      ld(geo<S>(B, 3 + 3 * DIR), x), ld(geo<S>(B, 0), x), f);
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
                 sq[lds_at<DIR, TA, TB>(c, s, a)] = q[c];
@@ -335,8 +346,8 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
                 roe_average(ra, ua, va, wa, enthalpy(ra, ua, va, wa, pa), rb, ub, vb, wb, enthalpy(rb, ub, vb, wb, pb), roe);
             }
             S llf[5];
-            face_llf(q, roe, S(ld(B.kf[DIR][0], gl)), S(ld(B.kf[DIR][1], gl)), S(ld(B.kf[DIR][2], gl)), S(ld(B.df[DIR], gl)),
-                     S(B.eps), llf);
+            face_llf(q, roe, ld(geo<S>(B, 10 + 3 * DIR), gl), ld(geo<S>(B, 11 + 3 * DIR), gl), ld(geo<S>(B, 12 + 3 * DIR), gl),
+                     ld(geo<S>(B, 19 + DIR), gl), S(B.eps), llf);
 #pragma unroll
             for (int c = 0; c < 5; ++c)
                 st((S *)B.FH[DIR][c], gl,
@@ -865,6 +876,20 @@ void count_nonfinite(const double *p, size_t n, unsigned *count, hipStream_t st)
 {
     const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_nonfinite, dim3(nb), dim3(256), 0, st, p, n, count);
+}
+
+__global__ void __launch_bounds__(256) k_geo32(const double *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = (float)src[q];
+}
+
+void geometry_f32(const BlockDev &B, hipStream_t st)
+{
+    if (!B.geo32) return;
+    const size_t n = (size_t)22 * B.n;
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_geo32, dim3(nb), dim3(256), 0, st, B.det, const_cast<float *>(B.geo32), n);
 }
 
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out)
