@@ -217,7 +217,13 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
 #pragma unroll
     for (int c = 0; c < 9; ++c) m[c] = ld(geo<S>(B, 1 + c), g);
     S fv[3][5];
-    const S mu = viscous_node(dUs, Uv, ld((const S *)B.P[1], g), ld((const S *)B.P[2], g), ld((const S *)B.P[3], g), m,
+    // the node velocity: away from the sides, from the stage vector already in registers (U_m /
+    // U_0, the primitives of P up to rounding) instead of three more loads; near the sides the
+    // side conditions set P apart from the stage vector, so it is read from P
+    S vel[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vel[c] = LOC ? ld((const S *)B.P[1 + c], g) : S(U[1 + c] * iU0);
+    const S mu = viscous_node(dUs, Uv, vel[0], vel[1], vel[2], m,
                               ld(geo<S>(B, 0), g), KSs, B.vp, fv);
 #pragma unroll
     for (int d = 0; d < 3; ++d)

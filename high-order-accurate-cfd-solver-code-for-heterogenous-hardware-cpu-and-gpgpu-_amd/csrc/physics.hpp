@@ -305,7 +305,9 @@ __device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5],
 
 // Viscous + MTS-SGS node fluxes (Viscous.cl:439-631) of one node.
 //   dU[c][d] = dU_c / d(zeta, eta, xi)_d of the det-scaled state U (Viscous.cl:285-435)
-//   U[5]     = U at the node; u, v, w = FLOW velocities at the node (Viscous.cl:146-154)
+//   U[5]     = U at the node; u, v, w = the velocity at the node (Viscous.cl:146-154 reads it from
+//              FLOW; the block kernel passes U_1..3 / U_0 of the same stage vector away from the
+//              sides of a block)
 //   m[9]     = node metrics zeta_x..xi_z; D = det
 //   KS[3]    = MTS scale-similarity terms KS_x, KS_y, KS_z (Viscous.cl:493-495)
 // Writes the contravariant det-scaled viscous fluxes fvis[0..2] = Fv, Ev, Gv and returns mu.
