@@ -37,8 +37,6 @@ FP64_PEAK_TFLOPS = 78.6
 # VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 ALG_BYTES_PER_CELL_UPDATE = 848  # minimal FP64 SoA traffic of one RK3 step (3 stages)
-# k_face per face: reads the 5 primitives P and writes the 5 face fluxes F-hat in the flux type
-# (8 B FP64, 4 B mixed) and reads 8 FP64 geometry values (metric row 3, det, face normal 3, face det)
 # Operations per node written out in the node kernels (counted from block_kernels.hip; the device
 # physics they call is counted by cfdx_op_count): viscous kernel stencil differences and MTS
 # similarity terms; update kernel face-flux differences, viscous divergence, mean RK stage and
@@ -60,7 +58,10 @@ def alg_ops_per_cell_update(counts: dict, viscous: bool) -> float:
 
 
 def face_bytes_per_node(precision: str) -> int:
-    return 64 + 10 * (8 if precision == "fp64" else 4)
+    """Algorithmic HBM bytes of one face of a face sweep: primitives 5, metric row 3, det, face
+    normal 3, face det read and F-hat 5 written, all in the flux type (a mixed block reads the
+    float copies of its geometry): 18 values."""
+    return 18 * (8 if precision == "fp64" else 4)
 
 
 def _import_pkg():
