@@ -167,9 +167,17 @@ def cpu_baseline(cfdx, max_seconds: float = 15.0):
     orc.steps(case, steps, 0.1, dt, prm, work)
     t3 = time.perf_counter()
     cups = n ** 3 * steps / (t3 - t2)
+    # the same block on one thread, a few steps
+    orc.set_threads(1)
+    t4 = time.perf_counter()
+    orc.steps(case, 2, 0.1, dt, prm, work)
+    t5 = time.perf_counter()
+    orc.set_threads(cores)
     return {"value": cups, "unit": "cell-updates/s", "cores": cores, "kind": kind,
             "sample": f"{steps} RK3 steps of a {n}^3 curvilinear TGV block (same kernels, FP64, OpenMP over nodes)",
-            "seconds": round(t3 - t2, 2)}
+            "seconds": round(t3 - t2, 2),
+            "single_thread": {"value": n ** 3 * 2 / (t5 - t4), "cores": 1, "sample": f"2 RK3 steps of the same {n}^3 block",
+                              "seconds": round(t5 - t4, 2)}}
 
 
 def make_block(cfdx, cdef, precision, prm, ctx=None, host_case=None):
