@@ -398,8 +398,8 @@ SERVE_MAX_NODES = 5_000_000
 
 
 def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kind: str = "airfoil"):
-    """A bounded sample of a Poisson request stream served by continuous batching (mixed) and by
-    plain FP64 one request at a time, on the same stream."""
+    """A bounded sample of a Poisson request stream served by continuous batching (mixed; arrival
+    order, and least work first) and by plain FP64 one request at a time, on the same stream."""
     import numpy as np
     from cfdx import serve
     if kind == "airfoil":
@@ -411,11 +411,13 @@ def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kin
         reqs = serve.periodic_stream(nreq, rate, seed=seed, nsteps=nsteps)
         out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, grids 128x64x16..384x128x32, "
                          "periodic TGV / vortex (curvilinear map), Mach U(0.1,0.6), Re U(500,5000)"}
-    for name, prec, batched in (("batched_mixed", cfdx.MIXED, True), ("plain_fp64", cfdx.FP64, False)):
+    runs = (("batched_mixed", cfdx.MIXED, True, "fifo"), ("batched_mixed_sjf", cfdx.MIXED, True, "sjf"),
+            ("plain_fp64", cfdx.FP64, False, "fifo"))
+    for name, prec, batched, policy in runs:
         # batched: at most SERVE_MAX_NODES nodes in flight (enough to fill the GPU; the earlier
-        # requests finish first under load)
+        # requests finish first under load); "sjf" admits the queued request with the least work
         srv = serve.Server(precision=prec, batched=batched, chunk=10, graph=True, keep_flow=True,
-                           max_nodes=SERVE_MAX_NODES if batched else None)
+                           max_nodes=SERVE_MAX_NODES if batched else None, policy=policy)
         res = srv.run(reqs, clock="wall")
         rep = serve.report(reqs, res)
         rep["peak_active"] = srv.peak_active

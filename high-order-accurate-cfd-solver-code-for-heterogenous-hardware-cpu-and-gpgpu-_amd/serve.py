@@ -4,7 +4,8 @@ Each request is one block (grid size, map, initial state, Mach, Re, number of RK
 server keeps the state of every admitted request resident in HBM and steps all of them in the
 same batched launches (continuous batching): between chunks of steps it retires finished
 requests (result copied to the host, block freed) and admits queued ones (first come, first
-served) while the device has free memory (hipMemGetInfo minus a reserve, optionally capped by a
+served, or with policy "sjf" the queued request with the least work first) while the device has
+free memory (hipMemGetInfo minus a reserve, optionally capped by a
 byte budget) and, optionally, while the nodes in flight stay under a cap. A request's
 arithmetic does not depend on what else is in the batch, so its result is the same alone and
 in the mix (the tests check bitwise equality).
@@ -18,6 +19,7 @@ in RK3 steps executed by the server, which makes a schedule reproducible for tes
 """
 from __future__ import annotations
 
+import bisect
 import collections
 import concurrent.futures
 import dataclasses
@@ -132,7 +134,13 @@ class Server:
     def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
                  budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
                  grid_cap: int = 0, batched: bool = True, keep_flow: bool = True, prep_workers: int = 8,
-                 max_nodes: int | None = None, pipelined: bool = True):
+                 max_nodes: int | None = None, pipelined: bool = True, policy: str = "fifo"):
+        if policy not in ("fifo", "sjf"):
+            raise ValueError(f"policy {policy!r}: 'fifo' or 'sjf'")
+        # admission order of the queued requests: arrival ("fifo") or least work, nodes x steps,
+        # first ("sjf": shorter median latency under load; a large request waits while smaller
+        # ones keep arriving)
+        self.policy = policy
         self.ctx = ctx or Ctx()
         # block set-up and result copies (same device, another stream)
         self.io = Ctx(self.ctx.device) if pipelined else self.ctx
@@ -168,7 +176,8 @@ class Server:
         if clock not in ("wall", "steps"):
             raise ValueError(clock)
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        queue: collections.deque[Request] = collections.deque()
+        queue: list[Request] = []          # arrived, not admitted; admission takes queue[0]
+        order = (lambda r: (r.nodes * r.nsteps, r.arrival, r.rid)) if self.policy == "sjf" else None
         active: dict[int, dict] = {}      # in the batch
         admitted: list[dict] = []          # set up, joining the batch at the next chunk boundary
         results: dict[int, Result] = {}
@@ -189,10 +198,13 @@ class Server:
             t = now()
             while pending and pending[0].arrival <= t:
                 req = pending.popleft()
-                queue.append(req)
+                if order is None:
+                    queue.append(req)
+                else:
+                    bisect.insort(queue, req, key=order)
                 if req.grid == "cgrid":
                     preps[req.rid] = self.pool.submit(prepare, req)
-            # admission: FIFO while memory, slots and the node cap allow, once the host set-up is
+            # admission in queue order while memory, slots and the node cap allow, once the host set-up is
             # ready (waited for when nothing else could run); the block is built on the set-up
             # stream while a chunk may be running
             busy = bool(active or admitted)
@@ -202,7 +214,7 @@ class Server:
                 fut = preps.get(req.rid)
                 if fut is not None and not fut.done() and (busy or inflight):
                     break
-                queue.popleft()
+                queue.pop(0)
                 prep = preps.pop(req.rid).result() if fut is not None else None
                 b, dt = make_block(req, self.precision, self.io, prep)
                 nb = req.bytes(self.precision)

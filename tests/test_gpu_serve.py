@@ -95,6 +95,31 @@ def test_serving_alone_vs_mix(ctx, precision, graph):
     assert rep["p99_s"] >= rep["p50_s"] > 0
 
 
+def test_serving_sjf_admits_least_work_first(ctx):
+    """policy "sjf": of requests queued together, the one with the least work (nodes x steps) is
+    admitted first; results stay bitwise the request alone."""
+    reqs = small_stream()
+    for r in reqs:
+        r.arrival = 0
+    # the largest block's bytes: about one block at a time, so admission order shows
+    big = max(cfdx.Block.bytes_for(r.ni, r.nj, r.nk, cfdx.FP64) for r in reqs)
+    srv = serve.Server(precision=cfdx.FP64, ctx=ctx, reserve_bytes=0, budget_bytes=big, chunk=4, policy="sjf")
+    res = srv.run(reqs, clock="steps")
+    srv.close()
+    work = {r.rid: r.nodes * r.nsteps for r in reqs}
+    for a in reqs:
+        for b in reqs:
+            if work[a.rid] < work[b.rid]:
+                assert res[a.rid].admitted <= res[b.rid].admitted, (a.rid, b.rid)
+    assert len({res[r.rid].admitted for r in reqs}) >= 3
+    for r in reqs:
+        alone = serve.run_alone(r, cfdx.FP64, ctx)
+        for f in FIELDS:
+            assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
+    with pytest.raises(ValueError):
+        serve.Server(precision=cfdx.FP64, ctx=ctx, policy="lifo")
+
+
 def test_serving_wall_clock(ctx):
     reqs = small_stream()
     for r in reqs:
