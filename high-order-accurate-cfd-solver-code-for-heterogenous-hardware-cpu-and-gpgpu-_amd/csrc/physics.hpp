@@ -132,39 +132,43 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     const T h_c = c3o8 * W[2] + c3o4 * W[3] - c1o8 * W[4];    // pos q1 = neg q0 (i, i+1, i+2)
     const T h_l = -c1o8 * W[1] + c3o4 * W[2] + c3o8 * W[3];   // pos q2 = neg q1 (i-1, i, i+1)
     const T h_ll = c3o8 * W[0] - c5o4 * W[1] + c15o8 * W[2];  // neg q2 (i-2, i-1, i)
-    const T c13 = T(13.0 / 12.0), c14 = T(0.25);
+    // IS = 13/12 s^2 + 1/4 t^2 is formed as 13/12 s^2 + (t/2)^2 with t/2 taken by halving the
+    // coefficients (exact scalings by 1/2: the same bits, one multiply fewer per indicator)
+    const T c13 = T(13.0 / 12.0);
     const T s0 = W[0] - T(2) * W[1] + W[2];
     const T s1 = W[1] - T(2) * W[2] + W[3];
     const T s2 = W[2] - T(2) * W[3] + W[4];
     const T s3 = W[3] - T(2) * W[4] + W[5];
     // pos indicators
-    const T t0 = T(3) * W[3] - T(4) * W[4] + W[5];
-    const T t1 = W[2] - W[4];
-    const T t2 = W[1] - T(4) * W[2] + T(3) * W[3];
-    const T ISp0 = c13 * s3 * s3 + c14 * t0 * t0;
-    const T ISp1 = c13 * s2 * s2 + c14 * t1 * t1;
-    const T ISp2 = c13 * s1 * s1 + c14 * t2 * t2;
+    const T t0 = T(1.5) * W[3] - T(2) * W[4] + T(0.5) * W[5];
+    const T t1 = T(0.5) * (W[2] - W[4]);
+    const T t2 = T(0.5) * W[1] - T(2) * W[2] + T(1.5) * W[3];
+    const T ISp0 = c13 * s3 * s3 + t0 * t0;
+    const T ISp1 = c13 * s2 * s2 + t1 * t1;
+    const T ISp2 = c13 * s1 * s1 + t2 * t2;
     // neg indicators
-    const T u0 = T(3) * W[2] - T(4) * W[3] + W[4];
-    const T u1 = W[1] - W[3];
-    const T u2 = W[0] - T(4) * W[1] + T(3) * W[2];
-    const T ISm0 = c13 * s2 * s2 + c14 * u0 * u0;
-    const T ISm1 = c13 * s1 * s1 + c14 * u1 * u1;
-    const T ISm2 = c13 * s0 * s0 + c14 * u2 * u2;
+    const T u0 = T(1.5) * W[2] - T(2) * W[3] + T(0.5) * W[4];
+    const T u1 = T(0.5) * (W[1] - W[3]);
+    const T u2 = T(0.5) * W[0] - T(2) * W[1] + T(1.5) * W[2];
+    const T ISm0 = c13 * s2 * s2 + u0 * u0;
+    const T ISm1 = c13 * s1 * s1 + u1 * u1;
+    const T ISm2 = c13 * s0 * s0 + u2 * u2;
     if constexpr (!std::is_same<T, float>::value) {
-        // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2: one division serves both sides
+        // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2: one division serves both sides;
+        // the weights of each side are taken 16 times larger (d = 1, 10, 5): the factor cancels
+        // exactly in the quotients
         T np_, dp_, nn_, dn_;
         {
             T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
             a0 *= a0; a1 *= a1; a2 *= a2;
-            const T w0 = T(1.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(5.0 / 16.0) * (a0 * a1);
+            const T w0 = a1 * a2, w1 = T(10) * (a0 * a2), w2 = T(5) * (a0 * a1);
             np_ = w0 * h_r + w1 * h_c + w2 * h_l;
             dp_ = w0 + w1 + w2;
         }
         {
             T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
             a0 *= a0; a1 *= a1; a2 *= a2;
-            const T w0 = T(5.0 / 16.0) * (a1 * a2), w1 = T(5.0 / 8.0) * (a0 * a2), w2 = T(1.0 / 16.0) * (a0 * a1);
+            const T w0 = T(5) * (a1 * a2), w1 = T(10) * (a0 * a2), w2 = a0 * a1;
             nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
             dn_ = w0 + w1 + w2;
         }
@@ -286,9 +290,10 @@ __device__ __forceinline__ void face_llf(const T (&q)[6][5], const T (&roe)[5], 
 template <typename T>
 __device__ __forceinline__ T midpoint_correction(T f0, T f1, T f2, T f3, T f4, T f5)
 {
-    const T d2 = T(1.0 / 48.0) * (T(-5) * f0 + T(39) * f1 - T(34) * f2 - T(34) * f3 + T(39) * f4 - T(5) * f5);
-    const T d4 = T(0.5) * (f0 - T(3) * f1 + T(2) * f2 + T(2) * f3 - T(3) * f4 + f5);
-    return -T(1.0 / 24.0) * d2 + T(7.0 / 5760.0) * d4;
+    // d2 = (-5 f0 + 39 f1 - 34 f2 - 34 f3 + 39 f4 - 5 f5) / 48,
+    // d4 = (f0 - 3 f1 + 2 f2 + 2 f3 - 3 f4 + f5) / 2; -d2 / 24 + 7 d4 / 5760 collects to
+    // (57 (f0 + f5) - 411 (f1 + f4) + 354 (f2 + f3)) / 11520
+    return T(57.0 / 11520.0) * (f0 + f5) - T(411.0 / 11520.0) * (f1 + f4) + T(354.0 / 11520.0) * (f2 + f3);
 }
 
 // Face flux F-hat = LLF + midpoint correction from node fluxes nf at offsets -2..+3.
