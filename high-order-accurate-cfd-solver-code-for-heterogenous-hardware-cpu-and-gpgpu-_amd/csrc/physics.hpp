@@ -153,38 +153,34 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     const T ISm0 = c13 * s2 * s2 + u0 * u0;
     const T ISm1 = c13 * s1 * s1 + u1 * u1;
     const T ISm2 = c13 * s0 * s0 + u2 * u2;
+    // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2, so each side needs one division; the
+    // weights of each side are taken 16 times larger (d = 1, 10, 5): the factor cancels exactly
+    // in the quotients
+    T np_, dp_, nn_, dn_;
+    {
+        T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
+        a0 *= a0; a1 *= a1; a2 *= a2;
+        const T w0 = a1 * a2, w1 = T(10) * (a0 * a2), w2 = T(5) * (a0 * a1);
+        np_ = w0 * h_r + w1 * h_c + w2 * h_l;
+        dp_ = w0 + w1 + w2;
+    }
+    {
+        T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
+        a0 *= a0; a1 *= a1; a2 *= a2;
+        const T w0 = T(5) * (a1 * a2), w1 = T(10) * (a0 * a2), w2 = a0 * a1;
+        nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
+        dn_ = w0 + w1 + w2;
+    }
     if constexpr (!std::is_same<T, float>::value) {
-        // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2: one division serves both sides;
-        // the weights of each side are taken 16 times larger (d = 1, 10, 5): the factor cancels
-        // exactly in the quotients
-        T np_, dp_, nn_, dn_;
-        {
-            T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
-            a0 *= a0; a1 *= a1; a2 *= a2;
-            const T w0 = a1 * a2, w1 = T(10) * (a0 * a2), w2 = T(5) * (a0 * a1);
-            np_ = w0 * h_r + w1 * h_c + w2 * h_l;
-            dp_ = w0 + w1 + w2;
-        }
-        {
-            T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
-            a0 *= a0; a1 *= a1; a2 *= a2;
-            const T w0 = T(5) * (a1 * a2), w1 = T(10) * (a0 * a2), w2 = a0 * a1;
-            nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
-            dn_ = w0 + w1 + w2;
-        }
+        // one reciprocal serves both sides
         const T r = pinv_(dp_ * dn_);
         pos = np_ * (dn_ * r);
         neg = nn_ * (dp_ * r);
     } else {
-        // float: the products above leave the float range on smooth data (eps^8 ~ 1e-56);
-        // w_k = d_k / (eps + IS_k)^2 with hardware reciprocals stays within it
-        auto side = [&](T i0, T i1, T i2, T d0, T d1, T d2, T q0, T q1, T q2) {
-            T a0 = eps + i0, a1 = eps + i1, a2 = eps + i2;
-            const T w0 = d0 * inv_(a0 * a0), w1 = d1 * inv_(a1 * a1), w2 = d2 * inv_(a2 * a2);
-            return (w0 * q0 + w1 * q1 + w2 * q2) * inv_(w0 + w1 + w2);
-        };
-        pos = side(ISp0, ISp1, ISp2, T(1.0 / 16.0), T(5.0 / 8.0), T(5.0 / 16.0), h_r, h_c, h_l);
-        neg = side(ISm0, ISm1, ISm2, T(5.0 / 16.0), T(5.0 / 8.0), T(1.0 / 16.0), h_c, h_l, h_ll);
+        // float: the weights of smooth data are ~eps^4 ~ 1e-28, within the float range, but the
+        // product of the two sides' sums (~eps^8) is not: one hardware reciprocal per side
+        pos = np_ * inv_(dp_);
+        neg = nn_ * inv_(dn_);
     }
 }
 
