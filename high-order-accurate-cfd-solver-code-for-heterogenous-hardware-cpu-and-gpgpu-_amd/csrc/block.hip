@@ -236,11 +236,17 @@ uint32_t mon_triples(const BlockDev &B, const cfdx::blk::TileCfg &t)
     return cfdx::blk::tiles_of(B, 0, t) * (uint32_t)(tx * ty / 64) + (cfdx::blk::shell_count(B) + 63) / 64;
 }
 
+cfdx::blk::Overlap overlap_of(const cfdx_ctx *ctx)
+{
+    return cfdx::blk::Overlap{ctx->aux, ctx->fork, ctx->join, (uint32_t)std::max(ctx->n_cu, 1)};
+}
+
 int run_stage(cfdx_block *b, int mode)
 {
     const LaunchList L = single_list(b);
     if (mode == 2) b->mon_count = mon_triples(b->B, b->tiles);
-    cfdx::blk::stage(L, b->desc.precision, mode, b->ctx->stream, b->B.viscous != 0, b->timing ? b->ev : nullptr);
+    const cfdx::blk::Overlap ov = overlap_of(b->ctx);
+    cfdx::blk::stage(L, b->desc.precision, mode, b->ctx->stream, b->B.viscous != 0, b->timing ? b->ev : nullptr, &ov);
     b->launches += (b->B.viscous ? 5 : 4);
     CFDX_TRY(cfdx::check_launch("block stage"));
     if (b->timing) {
@@ -830,14 +836,17 @@ extern "C" int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph)
     CFDX_TRY(upload_batch(q));
     for (cfdx_block *b : q->blocks) b->mon_count = mon_triples(b->B, q->tiles);
     hipStream_t st = q->ctx->stream;
+    const cfdx::blk::Overlap ov = overlap_of(q->ctx);
     if (!use_graph) {
         for (int s = 0; s < nsteps; ++s)
-            for (int j = 0; j < 3; ++j) cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr);
+            for (int j = 0; j < 3; ++j) cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, &ov);
         return cfdx::check_launch("batch stage");
     }
     if (!q->exec) {
         CFDX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        for (int j = 0; j < 3; ++j) cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr);
+        // the second stream joins the capture through the fork event and is joined back before
+        // each update, so the graph holds the overlap too
+        for (int j = 0; j < 3; ++j) cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, &ov);
         hipError_t e = hipStreamEndCapture(st, &q->graph);
         if (e != hipSuccess) { q->graph = nullptr; return cfdx::hip_fail(e, "hipStreamEndCapture"); }
         e = hipGraphInstantiate(&q->exec, q->graph, nullptr, nullptr, 0);

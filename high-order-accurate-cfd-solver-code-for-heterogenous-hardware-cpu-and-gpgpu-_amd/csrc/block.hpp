@@ -91,7 +91,16 @@ struct LaunchList {
 // kernels (block_kernels.hip)
 // ev (optional): 6 events recorded before the viscous kernel, after it, after each face sweep
 // and after the update
-void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev);
+// ov (optional, ignored when timing with ev): the viscous node kernel runs on ov->aux, forked
+// from st before the face sweeps and joined back before the update, with at most ov->visc_grid
+// workgroups, so that it shares the CUs with the (VALU-bound) face sweeps
+struct Overlap {
+    hipStream_t aux;
+    hipEvent_t fork, join;
+    uint32_t visc_grid;
+};
+void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev,
+           const Overlap *ov = nullptr);
 // the float copies of the geometry of a mixed block (after every write of its double geometry)
 void geometry_f32(const BlockDev &B, hipStream_t st);
 // P and U0 of the owned nodes from double primitives staged component-major in owned order

@@ -789,14 +789,28 @@ void face_jk(const LaunchList &L, dim3 g, hipStream_t st)
 }
 
 template <typename S>
-void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev)
+void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev, const Overlap *ov)
 {
     auto rec = [&](int s) { if (ev) (void)hipEventRecord(ev[s], st); };
     auto grid = [&](uint32_t total) { return dim3(L.grid_cap ? std::min(total, L.grid_cap) : total); };
     rec(0);
-    if (any_visc) node_kernel<S>(L, grid(L.total[0]), st, true, mode);
+    // the viscous node kernel reads the stage vector, primitives and geometry and writes Fv and mu;
+    // the face sweeps read the primitives and geometry and write F-hat: independent until the
+    // update, so the viscous kernel may run beside the sweeps on the second stream (persistent,
+    // one workgroup per CU, leaving the CUs' remaining slots to the sweeps)
+    const bool overlap = ov && !ev && any_visc;
+    hipStream_t vs = st;
+    dim3 gv = grid(L.total[0]);
+    if (overlap) {
+        (void)hipEventRecord(ov->fork, st);
+        (void)hipStreamWaitEvent(ov->aux, ov->fork, 0);
+        vs = ov->aux;
+        gv.x = std::min(gv.x, ov->visc_grid);
+    }
+    if (any_visc) node_kernel<S>(L, gv, vs, true, mode);
     if (any_visc && L.any_cut)
-        hipLaunchKernelGGL(k_cut_viscous<S>, dim3((3 * L.bc_face_d[1] + 255) / 256, L.nblk), dim3(256), 0, st, L.blks);
+        hipLaunchKernelGGL(k_cut_viscous<S>, dim3((3 * L.bc_face_d[1] + 255) / 256, L.nblk), dim3(256), 0, vs, L.blks);
+    if (overlap) (void)hipEventRecord(ov->join, vs);
     rec(1);
     switch (L.tiles.face[0]) {
     case 1: launch_face_i<S, 1>(L, grid(L.total[1]), st); break;
@@ -808,15 +822,16 @@ void launch_stage(const LaunchList &L, int mode, hipStream_t st, bool any_visc, 
     rec(3);
     face_jk<S, 2>(L, grid(L.total[3]), st);
     rec(4);
+    if (overlap) (void)hipStreamWaitEvent(st, ov->join, 0);
     node_kernel<S>(L, grid(L.total[0]), st, false, mode);
     if (L.bc_face && mode <= 2) launch_bc<S>(L, mode, st);
     rec(5);
 }
 
-void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev)
+void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev, const Overlap *ov)
 {
-    if (precision == CFDX_MIXED) launch_stage<float>(L, mode, st, any_visc, ev);
-    else launch_stage<double>(L, mode, st, any_visc, ev);
+    if (precision == CFDX_MIXED) launch_stage<float>(L, mode, st, any_visc, ev, ov);
+    else launch_stage<double>(L, mode, st, any_visc, ev, ov);
 }
 
 void base(const BlockDev &B, int precision, hipStream_t st, const double *stage)

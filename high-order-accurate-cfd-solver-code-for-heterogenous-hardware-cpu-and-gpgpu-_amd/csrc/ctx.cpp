@@ -122,6 +122,16 @@ extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
             return cfdx::hip_fail(e, "hipEventCreate");
         }
     }
+    {
+        hipError_t e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess) {
+            cfdx::ctx_release(c);
+            return cfdx::hip_fail(e, "ctx_create second stream");
+        }
+    }
     *out = c;
     return CFDX_OK;
 }
@@ -143,6 +153,12 @@ int ctx_release(cfdx_ctx *ctx)
     }
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->aux) {
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipStreamDestroy(ctx->aux);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
     delete ctx;
     return rc;
 }

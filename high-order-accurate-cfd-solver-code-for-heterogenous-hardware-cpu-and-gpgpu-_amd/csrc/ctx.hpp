@@ -13,6 +13,11 @@ struct cfdx_ctx {
     hipStream_t stream = nullptr;
     bool owns_stream = false;
     hipEvent_t ev[8] = {};
+    // a second stream of the device for work of a stage that may overlap the main stream's
+    // (forked from and joined back to the main stream by the two events), and the device's CUs
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    int n_cu = 0;
     std::atomic<int> refs{1};
 };
 
