@@ -98,6 +98,7 @@ EXPORTS = {
     "cfdx_ctx_create": (C.c_int, [C.c_int, _vp, C.POINTER(_vp)]),
     "cfdx_ctx_destroy": (C.c_int, [_vp]),
     "cfdx_ctx_set_stream": (C.c_int, [_vp, _vp]),
+    "cfdx_ctx_set_overlap": (C.c_int, [_vp, C.c_int]),
     "cfdx_ctx_stream": (_vp, [_vp]),
     "cfdx_sync": (C.c_int, [_vp]),
     "cfdx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -404,6 +405,11 @@ class Ctx:
         f, t = _i64(0), _i64(0)
         _check(_lib.cfdx_mem_info(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+    def set_overlap(self, on: bool) -> None:
+        """Mixed-precision stages run the viscous kernel beside the face sweeps on a second stream
+        (cfdx_ctx_set_overlap; on by default)."""
+        _check(_lib.cfdx_ctx_set_overlap(self.h, 1 if on else 0))
 
     def pool_trim(self) -> None:
         """Frees the block memory kept for reuse on this device (cfdx_pool_trim)."""

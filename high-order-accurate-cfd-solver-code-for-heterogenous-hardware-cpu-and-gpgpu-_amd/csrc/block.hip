@@ -236,9 +236,17 @@ uint32_t mon_triples(const BlockDev &B, const cfdx::blk::TileCfg &t)
     return cfdx::blk::tiles_of(B, 0, t) * (uint32_t)(tx * ty / 64) + (cfdx::blk::shell_count(B) + 63) / 64;
 }
 
+// the overlap of a stage's viscous kernel with its face sweeps, for mixed-precision work on a
+// context that allows it (the FP64 sweeps hold 3 waves per SIMD at ~160 VGPRs, which leaves no
+// registers for a viscous wave beside them: measured no gain, so FP64 stages run in order)
 cfdx::blk::Overlap overlap_of(const cfdx_ctx *ctx)
 {
     return cfdx::blk::Overlap{ctx->aux, ctx->fork, ctx->join, (uint32_t)std::max(ctx->n_cu, 1)};
+}
+
+const cfdx::blk::Overlap *overlap_if(const cfdx_ctx *ctx, int precision, const cfdx::blk::Overlap &ov)
+{
+    return (ctx->overlap && precision == CFDX_MIXED) ? &ov : nullptr;
 }
 
 int run_stage(cfdx_block *b, int mode)
@@ -246,7 +254,8 @@ int run_stage(cfdx_block *b, int mode)
     const LaunchList L = single_list(b);
     if (mode == 2) b->mon_count = mon_triples(b->B, b->tiles);
     const cfdx::blk::Overlap ov = overlap_of(b->ctx);
-    cfdx::blk::stage(L, b->desc.precision, mode, b->ctx->stream, b->B.viscous != 0, b->timing ? b->ev : nullptr, &ov);
+    cfdx::blk::stage(L, b->desc.precision, mode, b->ctx->stream, b->B.viscous != 0, b->timing ? b->ev : nullptr,
+                     overlap_if(b->ctx, b->desc.precision, ov));
     b->launches += (b->B.viscous ? 5 : 4);
     CFDX_TRY(cfdx::check_launch("block stage"));
     if (b->timing) {
@@ -839,14 +848,16 @@ extern "C" int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph)
     const cfdx::blk::Overlap ov = overlap_of(q->ctx);
     if (!use_graph) {
         for (int s = 0; s < nsteps; ++s)
-            for (int j = 0; j < 3; ++j) cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, &ov);
+            for (int j = 0; j < 3; ++j)
+                cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, overlap_if(q->ctx, q->precision, ov));
         return cfdx::check_launch("batch stage");
     }
     if (!q->exec) {
         CFDX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         // the second stream joins the capture through the fork event and is joined back before
         // each update, so the graph holds the overlap too
-        for (int j = 0; j < 3; ++j) cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, &ov);
+        for (int j = 0; j < 3; ++j)
+            cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, overlap_if(q->ctx, q->precision, ov));
         hipError_t e = hipStreamEndCapture(st, &q->graph);
         if (e != hipSuccess) { q->graph = nullptr; return cfdx::hip_fail(e, "hipStreamEndCapture"); }
         e = hipGraphInstantiate(&q->exec, q->graph, nullptr, nullptr, 0);

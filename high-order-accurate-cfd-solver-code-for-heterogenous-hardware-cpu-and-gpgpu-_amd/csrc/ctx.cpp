@@ -132,6 +132,7 @@ extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
             return cfdx::hip_fail(e, "ctx_create second stream");
         }
     }
+    if (const char *v = std::getenv("CFDX_OVERLAP")) c->overlap = std::atoi(v) != 0;
     *out = c;
     return CFDX_OK;
 }
@@ -180,6 +181,13 @@ extern "C" int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream)
         CFDX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
         ctx->owns_stream = true;
     }
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_ctx_set_overlap(cfdx_ctx *ctx, int on)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    ctx->overlap = on != 0;
     return CFDX_OK;
 }
 

@@ -18,6 +18,7 @@ struct cfdx_ctx {
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     int n_cu = 0;
+    bool overlap = true;   // mixed-precision stages overlap the viscous kernel with the face sweeps
     std::atomic<int> refs{1};
 };
 
