@@ -41,7 +41,7 @@ ALG_BYTES_PER_CELL_UPDATE = 848  # minimal FP64 SoA traffic of one RK3 step (3 s
 # physics they call is counted by cfdx_op_count): viscous kernel stencil differences and MTS
 # similarity terms; update kernel face-flux differences, viscous divergence, mean RK stage and
 # primitive recovery.
-VISC_KERNEL_OPS = 118
+VISC_KERNEL_OPS = 121
 UPDATE_KERNEL_OPS_EULER = 30 + 70.0 / 3 + 20   # 70 / 3: the mean RK stage, 5 x {2, 6, 6} over the 3 stages
 UPDATE_KERNEL_OPS_VISC_DIV = 72
 

@@ -125,7 +125,7 @@ __global__ void k_opcount(Ops *out, ViscParams vp)
 //   visc_node: per direction and component the central difference (8 (f+1 - f-1) - (f+2 - f-2))
 //     / 12 = 5 operations (75 for 5 components x 3 directions); per direction the MTS
 //     scale-similarity term a = U/U0 - (weighted mean)/(weighted mean), a^2 = 14 operations
-//     (42); 1 / U0 = 1 (total 118).
+//     (42); 1 / U0 = 1; the node velocity U_m / U_0 away from the sides = 3 (total 121).
 //   update_node: face-flux differences 2 x 15 = 30; viscous divergence 4 components x 3
 //     directions x (5 + 1) = 72; RK stage 5 x {2, 6, 6} (modes 0, 1, 2; mean 4.67) = 23.3 per
 //     stage; primitive recovery (1/det, rho, 1/rho, u v w, e, p) = 20; at the last stage the RK
