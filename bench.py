@@ -454,7 +454,7 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=int, default=256, help="size of the reference copy kernel run (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs one GPU per process; --gpus {args.gpus} is not supported "
               "(the periodic block is single-GPU, multi-GPU halo exchange is design-only)", file=sys.stderr)
         return 2
     legs = {x for x in args.legs.split(",") if x}

@@ -35,7 +35,7 @@ extern "C" {
 #define CFDX_E_INVALID     -1   /* bad argument (null pointer, size, flag) */
 #define CFDX_E_HIP         -2   /* HIP runtime call failed */
 #define CFDX_E_NOMEM       -3   /* device allocation failed */
-#define CFDX_E_UNSUPPORTED -4   /* valid request this build does not implement */
+#define CFDX_E_UNSUPPORTED -4   /* valid request the library does not implement */
 #define CFDX_E_NONFINITE   -5   /* non-finite state detected by a checked step */
 #define CFDX_E_CAPACITY    -6   /* request exceeds a fixed capacity */
 
