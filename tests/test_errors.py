@@ -27,6 +27,21 @@ def test_block_bytes_refuses_bad_descriptors(d, words):
     assert e.value.code == cfdx.E_INVALID and words in str(e.value)
 
 
+def test_block_bytes_refuses_blocks_past_the_index_range():
+    """Kernels index a block's arrays with 32-bit byte offsets: 2^29 stored nodes or more are
+    refused with CFDX_E_CAPACITY; one node row fewer is accepted."""
+    big = desc(1024, 1024, 512, 7)                       # 2^29 nodes
+    with pytest.raises(cfdx.CfdxError) as e:
+        cfdx.Block.bytes_for(big.ni, big.nj, big.nk, big.precision, desc=big)
+    assert e.value.code == cfdx.E_CAPACITY and "2^29" in str(e.value)
+    ok = desc(1024, 1023, 512, 7)
+    assert cfdx.Block.bytes_for(ok.ni, ok.nj, ok.nk, ok.precision, desc=ok) > 1023 * 1024 * 512 * 520
+    # a bounded direction counts its ghost layers: 1018 owned + 6 ghosts = 1024 stored
+    bnd = desc(1018, 1024, 512, 6, (3, 3, 0, 0, 0, 0))
+    with pytest.raises(cfdx.CfdxError):
+        cfdx.Block.bytes_for(bnd.ni, bnd.nj, bnd.nk, bnd.precision, desc=bnd)
+
+
 def test_block_bytes_valid():
     d = desc(10, 9, 8, 4, (3, 2, 1, 1, 0, 0))
     n = 16 * 15 * 8
