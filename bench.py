@@ -246,6 +246,30 @@ def field_errors(a, b):
     return out
 
 
+def dropin_path(cfdx, case, mach, dt, prm, block_value, steps: int = 3):
+    """The headline case through the drop-in kernels (the reference's AoS records and its per-stage
+    U_Q -> Viscous -> Weno sequence, each node computing both of its faces), on the device: the
+    throughput the reference's own data layout and call sequence reach on this GPU, beside the
+    structured block path. Plain FP64 (its parity with the reference is tested separately)."""
+    import numpy as np
+    s = cfdx.DropInSolver(case)
+    ctx = s.ctx
+    s.step(1, mach, dt, prm)                  # loads the kernels' code
+    ctx.sync()
+    s = cfdx.DropInSolver(case, ctx=ctx)
+    ctx.record(2)
+    s.step(steps, mach, dt, prm)
+    ctx.record(3)
+    ctx.sync()
+    ms = ctx.elapsed_ms(2, 3)
+    n = case["Grid_P"].shape[0] - 1
+    finite = bool(np.isfinite(cfdx_matrix(s.flow()[1:])).all())
+    value = n * steps / (ms / 1e3)
+    return {"workload": "the headline case through cfdx_uq / cfdx_viscous / cfdx_weno (AoS, both faces per node)",
+            "steps": steps, "value": value, "unit": "cell-updates/s", "ms_per_step": ms / steps,
+            "block_path_over_dropin": block_value / value, "state_finite": finite}
+
+
 def one_gpu_share(cfdx, tune: bool, steps: int = 10, warmup: int = 2):
     """The C2 map and flow at 256^3, plain FP64: one GPU's share of a 512^3 run split 2x2x2 (the
     periodic wrap stands in for the neighbours' halo), set up on the device."""
@@ -440,9 +464,10 @@ def main() -> int:
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--legs", default="compare,ns256,c4,c3,serve",
-                    help="extra legs after the headline: compare (other precision, per-field errors), ns256 (256^3 "
-                         "FP64, one GPU's share of 512^3), c4, c3, serve (none: the headline only)")
+    ap.add_argument("--legs", default="compare,dropin,ns256,c4,c3,serve",
+                    help="extra legs after the headline: compare (other precision, per-field errors), dropin (the "
+                         "same case through the drop-in kernels), ns256 (256^3 FP64, one GPU's share of 512^3), c4, "
+                         "c3, serve (none: the headline only)")
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--serve-kind", default="airfoil", choices=["airfoil", "periodic"])
     ap.add_argument("--c4-steps", type=int, default=5)
@@ -573,6 +598,9 @@ def main() -> int:
             "field_errors": field_errors(mixed_flow, fp64_flow),
         }
         print(f"bench: other precision {o_value:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    if "dropin" in legs and "cgrid" not in cdef:
+        line["dropin_path"] = dropin_path(cfdx, case, mach, dt, prm, value)
+        print(f"bench: drop-in kernels {line['dropin_path']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
     del case
     block.close()
 
