@@ -1,7 +1,7 @@
 """The benchmark configurations at their full sizes (GPU).
 
-- C1 (64^3 isentropic vortex, Euler, Cartesian through the metric path): 3 RK3 steps against the
-  oracle, 1e-12 relative;
+- C1 (64^3 isentropic vortex, Euler, Cartesian through the metric path): its 100 RK3 steps
+  against the oracle, 1e-12 relative;
 - C2 (128^3 curvilinear Taylor-Green, MTS LES): one RK3 step against the oracle, 1e-12 relative
   (velocity components against the largest velocity);
   the right-hand side against the oracle to 1e-12 of the face-flux scale (at Mach 0.1 on this
@@ -10,9 +10,12 @@
   viscous node fluxes telescope: sum over nodes of L = 0 up to the rounding of the sums) in FP64
   and mixed;
 - C3 (384x128x32 NACA 0012 C-grid): 20 steps from the free stream in mixed precision against
-  FP64 (per-field relative L2 <= 1e-5) with a finite state.
+  FP64 (per-field relative L2 <= 1e-5) with a finite state;
+- C2's full 200 steps against the oracle (about two minutes of CPU oracle work on 16 cores): run
+  only with CFDX_LONG_TESTS=1.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -41,14 +44,14 @@ def c2_case():
 
 def test_c1_full_size_steps(ctx, orc):
     import pyoracle
-    print("C1 64^3 vortex, 3 steps vs oracle", flush=True)
+    print("C1 64^3 vortex, 100 steps vs oracle", flush=True)
     case = cfdx.periodic_case(64, 64, 64, grid="box", length=(10.0,) * 3, init="vortex", mach=0.5)
     dt = cfdx.cfl_dt(case, 0.5)
     ref = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in case.items()}
-    orc.steps(ref, 3, 0.5, dt, pyoracle.reference_params(reyl=math.inf, c_mts=0.0))
+    orc.steps(ref, 100, 0.5, dt, pyoracle.reference_params(reyl=math.inf, c_mts=0.0))
     b = cfdx.Block(64, 64, 64, ctx=ctx)
     b.set_case(case)
-    b.step(3, 0.5, dt, cfdx.euler_params())
+    b.step(100, 0.5, dt, cfdx.euler_params())
     err = cases.rel_err(cases.flow_matrix(b.get_flow()), cases.flow_matrix(ref["FLOW"][1:]))
     b.close()
     assert err.max() <= TOL, err
@@ -96,6 +99,28 @@ def test_c2_full_size_step_rhs_and_conservation(ctx, orc):
         total, scale = np.abs(L.sum(axis=1)), np.abs(L).sum(axis=1)
         print("conservation", precision, total / scale, flush=True)
         assert (total <= 1e-11 * scale).all(), (precision, total, scale)
+
+
+@pytest.mark.skipif(os.environ.get("CFDX_LONG_TESTS") != "1", reason="C2's 200 oracle steps take minutes; CFDX_LONG_TESTS=1")
+def test_c2_200_steps(ctx, orc):
+    import pyoracle
+    print("C2 128^3 TGV, 200 steps vs oracle", flush=True)
+    case = c2_case()
+    dt = cfdx.cfl_dt(case, 0.5)
+    ref = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in case.items()}
+    for chunk in range(10):   # 20 steps at a time, with a progress line
+        orc.steps(ref, 20, 0.1, dt, pyoracle.reference_params(mach=0.1, reyl=1600.0))
+        print(f"oracle: {20 * (chunk + 1)} steps", flush=True)
+    b = cfdx.Block(128, 128, 128, ctx=ctx)
+    b.set_case(case)
+    b.step(200, 0.1, dt, cfdx.default_params(mach=0.1, reyl=1600.0))
+    got, want = b.get_flow(), ref["FLOW"][1:]
+    b.close()
+    vel = np.sqrt(sum(want[c] ** 2 for c in "uvw")).max()
+    for f in ("rho", "u", "v", "w", "p"):
+        e = np.abs(got[f] - want[f]).max() / (vel if f in "uvw" else np.abs(want[f]).max())
+        print(f"C2 200 steps {f}: {e:.2e}", flush=True)
+        assert e <= TOL, (f, e)
 
 
 def test_c3_full_size_mixed_vs_fp64(ctx):
