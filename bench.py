@@ -477,6 +477,9 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
     ap.add_argument("--copy-mib", type=int, default=256, help="size of the reference copy kernel run (HBM counter scale)")
+    ap.add_argument("--sgs", default="mts", choices=["mts", "smagorinsky"],
+                    help="SGS model of the timed workload: the reference's mixed time scale (default) or Smagorinsky")
+    ap.add_argument("--c-s", dest="c_s", type=float, default=0.17, help="Smagorinsky constant")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the library runs one GPU per process; --gpus {args.gpus} is not supported "
@@ -490,6 +493,10 @@ def main() -> int:
     t_setup = time.perf_counter()
     # periodic workloads: the case built on the host (the same arrays the oracle would take)
     case = None if "cgrid" in cdef else cfdx.periodic_case(cdef["ni"], cdef["nj"], cdef["nk"], **cdef["kw"])
+    if args.sgs == "smagorinsky" and viscous:
+        # the Smagorinsky option (no reference counterpart; the reference's own model is MTS)
+        pkw = dict(pkw, sgs="smagorinsky", c_s=args.c_s)
+        spec["sgs"] = f"smagorinsky(C_s={args.c_s})"
     prm = cfdx.default_params(**pkw) if viscous else cfdx.euler_params()
     mach = cdef["kw"]["mach"]
     n = cdef["ni"] * cdef["nj"] * cdef["nk"]

@@ -39,7 +39,8 @@ class Params(C.Structure):
     """cfdx_params: the constants the reference hard-codes, plus precision / viscous switches."""
     _fields_ = [("mach", C.c_double), ("reyl", C.c_double), ("pr", C.c_double), ("free_t", C.c_double),
                 ("c_mts", C.c_double), ("c_t", C.c_double), ("eps", C.c_double),
-                ("precision", C.c_int32), ("viscous", C.c_int32)]
+                ("precision", C.c_int32), ("viscous", C.c_int32), ("sgs", C.c_int32), ("pad_", C.c_int32),
+                ("c_s", C.c_double)]
 
 
 BC_WALL, BC_FARFIELD, BC_OUTFLOW, BC_EXTRAP, BC_WALL_CUT = 1, 2, 3, 4, 5
@@ -202,10 +203,20 @@ def device_count() -> int:
     return n.value if rc == 0 else 0
 
 
+SGS_MTS, SGS_SMAGORINSKY = 0, 1
+
+
 def default_params(**kw) -> Params:
+    """cfdx_params_default with the given fields replaced (names of cfdx_params; sgs="smagorinsky"
+    or "mts" is accepted for the SGS switch)."""
     p = Params()
     _lib.cfdx_params_default(C.byref(p))
+    names = {f[0] for f in Params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"default_params: no parameter {k!r}")
+        if k == "sgs" and isinstance(v, str):
+            v = {"mts": SGS_MTS, "smagorinsky": SGS_SMAGORINSKY}[v]
         setattr(p, k, v)
     return p
 

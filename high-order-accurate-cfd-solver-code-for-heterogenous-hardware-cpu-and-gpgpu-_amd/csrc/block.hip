@@ -152,8 +152,12 @@ std::vector<uint8_t> loc_codes(const cfdx_block_desc *d)
 int check_params(const cfdx_params *p)
 {
     if (!p) return cfdx::fail(CFDX_E_INVALID, "null params");
-    if (!p->viscous && !(std::isinf(p->reyl) && p->c_mts == 0.0))
-        return cfdx::fail(CFDX_E_INVALID, "viscous=0 requires reyl=inf and c_mts=0 (else the RHS differs)");
+    if (p->sgs != CFDX_SGS_MTS && p->sgs != CFDX_SGS_SMAGORINSKY)
+        return cfdx::fail(CFDX_E_INVALID, "sgs %d is neither CFDX_SGS_MTS nor CFDX_SGS_SMAGORINSKY", p->sgs);
+    if (!(p->c_s >= 0)) return cfdx::fail(CFDX_E_INVALID, "c_s must be >= 0");
+    const bool sgs_off = p->sgs == CFDX_SGS_MTS ? p->c_mts == 0.0 : p->c_s == 0.0;
+    if (!p->viscous && !(std::isinf(p->reyl) && sgs_off))
+        return cfdx::fail(CFDX_E_INVALID, "viscous=0 requires reyl=inf and the SGS model off (else the RHS differs)");
     if (!(p->eps > 0)) return cfdx::fail(CFDX_E_INVALID, "eps must be > 0");
     return CFDX_OK;
 }
@@ -165,7 +169,7 @@ void configure(cfdx_block *b, double mach, double del_t, const cfdx_params *p)
     b->B.mach = mach;
     b->B.eps = p->eps;
     b->B.viscous = p->viscous ? 1 : 0;
-    b->B.vp = cfdx::dev::make_visc_params(p->mach, p->reyl, p->pr, p->free_t, p->c_mts, p->c_t);
+    b->B.vp = cfdx::dev::make_visc_params(p->mach, p->reyl, p->pr, p->free_t, p->c_mts, p->c_t, p->sgs, p->c_s);
     b->last_mach = mach;
     b->configured = true;
     if (std::memcmp(&old, &b->B, sizeof old) != 0) ++b->version;

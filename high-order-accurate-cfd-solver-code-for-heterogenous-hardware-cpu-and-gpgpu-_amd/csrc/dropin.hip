@@ -221,7 +221,7 @@ cfdx::dev::ViscParams visc_params(const cfdx_params *prm)
 {
     cfdx_params d;
     if (!prm) { cfdx_params_default(&d); prm = &d; }
-    return cfdx::dev::make_visc_params(prm->mach, prm->reyl, prm->pr, prm->free_t, prm->c_mts, prm->c_t);
+    return cfdx::dev::make_visc_params(prm->mach, prm->reyl, prm->pr, prm->free_t, prm->c_mts, prm->c_t, prm->sgs, prm->c_s);
 }
 
 }  // namespace

@@ -26,6 +26,9 @@ extern "C" void cfdx_params_default(cfdx_params *p)
     p->eps = 10e-8;     // Weno.cl:59
     p->precision = CFDX_FP64;
     p->viscous = 1;
+    p->sgs = CFDX_SGS_MTS;
+    p->pad_ = 0;
+    p->c_s = 0.17;
 }
 
 extern "C" int cfdx_geom_periodic_table(int32_t ni, int32_t nj, int32_t nk, const double *xyz, cfdx_grid *G)

@@ -137,7 +137,8 @@ extern "C" int cfdx_op_count(cfdx_ctx *ctx, const cfdx_params *params, int64_t o
     cfdx_params d;
     if (!params) { cfdx_params_default(&d); params = &d; }
     const cfdx::dev::ViscParams vp =
-        cfdx::dev::make_visc_params(params->mach, params->reyl, params->pr, params->free_t, params->c_mts, params->c_t);
+        cfdx::dev::make_visc_params(params->mach, params->reyl, params->pr, params->free_t, params->c_mts, params->c_t,
+                                    params->sgs, params->c_s);
     cfdx::opc::Ops *dev = nullptr;
     CFDX_HIP(hipMalloc((void **)&dev, 4 * sizeof(cfdx::opc::Ops)), "op_count hipMalloc");
     hipLaunchKernelGGL(cfdx::opc::k_opcount, dim3(1), dim3(1), 0, ctx->stream, dev, vp);

@@ -304,7 +304,8 @@ __device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5],
         out[c] = TO(llf[c] + midpoint_correction(nf[0][c], nf[1][c], nf[2][c], nf[3][c], nf[4][c], nf[5][c]));
 }
 
-// Viscous + MTS-SGS node fluxes (Viscous.cl:439-631) of one node.
+// Viscous + SGS node fluxes (Viscous.cl:439-631, its mixed-time-scale model or the Smagorinsky
+// option) of one node.
 //   dU[c][d] = dU_c / d(zeta, eta, xi)_d of the det-scaled state U (Viscous.cl:285-435)
 //   U[5]     = U at the node; u, v, w = the velocity at the node (Viscous.cl:146-154 reads it from
 //              FLOW; the block kernel passes U_1..3 / U_0 of the same stage vector away from the
@@ -322,11 +323,15 @@ struct ViscParams {
     double tfac;         // 1.4 M^2 0.4
     double kQ;           // -1 / (1.4 0.4 M^2)
     double inv_ct;       // 1 / C_T
+    double cs2;          // Smagorinsky: C_s^2
+    int sgs;             // 0: mixed time scale (the reference); 1: Smagorinsky
+    double kheat_t;      // Smagorinsky heat flux: 1.4 / Pr
 };
 
-inline ViscParams make_visc_params(double mach, double reyl, double pr, double free_t, double c_mts, double c_t)
+inline ViscParams make_visc_params(double mach, double reyl, double pr, double free_t, double c_mts, double c_t,
+                                   int sgs = 0, double c_s = 0.0)
 {
-    ViscParams v{mach, reyl, pr, free_t, c_mts, c_t, 0, 0, 0, 0, 0, 0, 0};
+    ViscParams v{mach, reyl, pr, free_t, c_mts, c_t, 0, 0, 0, 0, 0, 0, 0, c_s * c_s, sgs, 1.4 / pr};
     v.inv_reyl = 1.0 / reyl;
     v.kheat = 1.4 / (pr * reyl);
     v.suth = 110.4 / free_t;
@@ -392,26 +397,45 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     const T mod = sqrt_(T(2) * ((sxy * sxy + syz * syz + sxz * sxz) + T(2.0 / 3.0) * (sxx * sxx + syy * syy + szz * szz)));
     const T DEL = inv_(cbrt_(D));
     T eddy[3];
+    if (vp.sgs == 1) {
+        // Smagorinsky (no reference counterpart): nu_t = (C_s Delta)^2 |S| with Delta = det^(1/3) and
+        // |S| = sqrt(2 S_ij S_ij), the same in every direction. The SGS stress below is
+        // -2 U0 eddy S_ij with U0 = det rho, so eddy = nu_t / det = C_s^2 |S| det^(-1/3) = C_s^2 |S| DEL.
+        const T s2 = T(2) * (sxx * sxx + syy * syy + szz * szz) + T(4) * (sxy * sxy + syz * syz + sxz * sxz);
+        const T e = T(vp.cs2) * DEL * sqrt_(s2);
+        eddy[0] = e; eddy[1] = e; eddy[2] = e;
+    } else {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T sk = sqrt_(KS[d]);
-        if (mod != T(0)) {
-            // C_MTS / (1 + R / C_T) * DEL * sk with R = DEL |S| / sk, written with one division
-            // (sk = 0 gives R = inf and a zero eddy viscosity either way)
-            eddy[d] = div_(C_MTS * DEL * sk * sk, sk + DEL * mod * T(vp.inv_ct));
-        } else {
-            eddy[d] = C_MTS * DEL * sk;
+        for (int d = 0; d < 3; ++d) {
+            const T sk = sqrt_(KS[d]);
+            if (mod != T(0)) {
+                // C_MTS / (1 + R / C_T) * DEL * sk with R = DEL |S| / sk, written with one division
+                // (sk = 0 gives R = inf and a zero eddy viscosity either way)
+                eddy[d] = div_(C_MTS * DEL * sk * sk, sk + DEL * mod * T(vp.inv_ct));
+            } else {
+                eddy[d] = C_MTS * DEL * sk;
+            }
         }
     }
     // SGS stresses scale with U0 = det * rho (Viscous.cl:529-535)
     const T m2U = T(-2) * U[0];
     const T Sxy = m2U * eddy[0] * sxy, Syz = m2U * eddy[1] * syz, Sxz = m2U * eddy[2] * sxz;
     const T Sxx = m2U * eddy[0] * sxx, Syy = m2U * eddy[1] * syy, Szz = m2U * eddy[2] * szz;
-    const T kQ = T(vp.kQ);
-    const T Qx = kQ * eddy[0] * ex, Qy = kQ * eddy[1] * ey, Qz = kQ * eddy[2] * ez;
-    const T DX = (Sxx + Sxy + Sxz) * u, DY = (Sxy + Syy + Syz) * v, DZ = (Sxz + Syz + Szz) * w;
-    // H = Q + DEL and D = 0.4 Q (Viscous.cl:549-557)
-    const T Hx = Qx + DX + T(0.4) * Qx, Hy = Qy + DY + T(0.4) * Qy, Hz = Qz + DZ + T(0.4) * Qz;
+    T Hx, Hy, Hz;
+    if (vp.sgs == 1) {
+        // Smagorinsky: the molecular forms with rho nu_t (= U0 eddy) for mu / Re: heat flux
+        // -1.4 rho nu_t / Pr grad e, and the work of the SGS stresses
+        const T kS = -(T(vp.kheat_t) * (U[0] * eddy[0]));
+        Hx = kS * ex + (u * Sxx + v * Sxy + w * Sxz);
+        Hy = kS * ey + (u * Sxy + v * Syy + w * Syz);
+        Hz = kS * ez + (u * Sxz + v * Syz + w * Szz);
+    } else {
+        const T kQ = T(vp.kQ);
+        const T Qx = kQ * eddy[0] * ex, Qy = kQ * eddy[1] * ey, Qz = kQ * eddy[2] * ez;
+        const T DX = (Sxx + Sxy + Sxz) * u, DY = (Sxy + Syy + Syz) * v, DZ = (Sxz + Syz + Szz) * w;
+        // H = Q + DEL and D = 0.4 Q (Viscous.cl:549-557)
+        Hx = Qx + DX + T(0.4) * Qx; Hy = Qy + DY + T(0.4) * Qy; Hz = Qz + DZ + T(0.4) * Qz;
+    }
     // Viscous.cl:576-595 Cartesian viscous fluxes
     const T tF[5] = {T(0), -txx + Sxx, -txy + Sxy, -txz + Sxz, (-u * txx - v * txy - w * txz) + qx + Hx};
     const T tE[5] = {T(0), -txy + Sxy, -tyy + Syy, -tyz + Syz, (-u * txy - v * tyy - w * tyz) + qy + Hy};

@@ -76,10 +76,19 @@ typedef struct {
     double c_t;       /* Viscous.cl:491 — MTS time-scale constant (ref 10) */
     double eps;       /* Weno.cl:59 — WCNS epsilon "10e-8" (ref 1e-7) */
     int32_t precision;/* CFDX_FP64 or CFDX_MIXED (structured blocks only) */
-    int32_t viscous;  /* 1 = full RHS; 0 = Euler (only when reyl is infinite and c_mts == 0) */
+    int32_t viscous;  /* 1 = full RHS; 0 = Euler (only when reyl is infinite and the SGS model is off) */
+    int32_t sgs;      /* CFDX_SGS_MTS: the reference's mixed-time-scale model (Viscous.cl:490-558);
+                         CFDX_SGS_SMAGORINSKY: SGS stress -2 rho nu_t S_ij, nu_t = (c_s Delta)^2 |S|,
+                         Delta = det^(1/3), |S| = sqrt(2 S_ij S_ij), the same SGS heat-flux form as
+                         the MTS model (no reference counterpart) */
+    int32_t pad_;
+    double c_s;       /* Smagorinsky constant (0.17); 0 disables that model */
 } cfdx_params;
+#define CFDX_SGS_MTS         0
+#define CFDX_SGS_SMAGORINSKY 1
 
-/* Fills the reference's constants, precision FP64, viscous on. */
+/* Fills the reference's constants, precision FP64, viscous on, the MTS model (c_s = 0.17 for the
+ * Smagorinsky option). */
 void cfdx_params_default(cfdx_params *p);
 
 /* ------------------------------------------------------------------------ */

@@ -210,7 +210,13 @@ void orc_viscous(int64_t n_launch, const orc_vec5 *U_C, orc_vec5 *F_C, orc_vec5 
         const double mod_stress = sqrt(2.0 * ((stress_xy * stress_xy + stress_yz * stress_yz + stress_xz * stress_xz) + (2.0 / 3.0) * (stress_xx * stress_xx + stress_yy * stress_yy + stress_zz * stress_zz)));
         const double DEL = (1.0 / cbrt(DET_I));
         double EDDY_x, EDDY_y, EDDY_z;
-        if (mod_stress != 0.0) {
+        if (prm->sgs == 1) {
+            /* Smagorinsky option: nu_t = (c_s det^(1/3))^2 |S|, |S| = sqrt(2 S_ij S_ij), carried as
+               nu_t / det because the SGS stresses below take U0_I = det * rho */
+            const double s2 = 2.0 * (stress_xx * stress_xx + stress_yy * stress_yy + stress_zz * stress_zz) +
+                              4.0 * (stress_xy * stress_xy + stress_yz * stress_yz + stress_xz * stress_xz);
+            EDDY_x = EDDY_y = EDDY_z = (prm->c_s * prm->c_s) * DEL * sqrt(s2);
+        } else if (mod_stress != 0.0) {
             const double Rx = (DEL * (mod_stress)) / sqrt(KS_x);
             const double Ry = (DEL * (mod_stress)) / sqrt(KS_y);
             const double Rz = (DEL * (mod_stress)) / sqrt(KS_z);
@@ -235,8 +241,20 @@ void orc_viscous(int64_t n_launch, const orc_vec5 *U_C, orc_vec5 *F_C, orc_vec5 
         const double DEL_X = TAU_SGS_XX * fl.u + TAU_SGS_XY * fl.u + TAU_SGS_XZ * fl.u;
         const double DEL_Y = TAU_SGS_XY * fl.v + TAU_SGS_YY * fl.v + TAU_SGS_YZ * fl.v;
         const double DEL_Z = TAU_SGS_XZ * fl.w + TAU_SGS_YZ * fl.w + TAU_SGS_ZZ * fl.w;
-        const double H_SGS_X = Q_x + DEL_X, H_SGS_Y = Q_y + DEL_Y, H_SGS_Z = Q_z + DEL_Z;
-        const double D_SGS_X = 0.4 * Q_x, D_SGS_Y = 0.4 * Q_y, D_SGS_Z = 0.4 * Q_z;
+        double H_SGS_X = Q_x + DEL_X, H_SGS_Y = Q_y + DEL_Y, H_SGS_Z = Q_z + DEL_Z;
+        double D_SGS_X = 0.4 * Q_x, D_SGS_Y = 0.4 * Q_y, D_SGS_Z = 0.4 * Q_z;
+        if (prm->sgs == 1) {
+            /* Smagorinsky option: heat flux -1.4 rho nu_t / Pr grad e (rho nu_t = U0_I * EDDY) and the
+               work of the SGS stresses */
+            const double kS = -(1.4 / prm->pr) * (U0_I * EDDY_x);
+            H_SGS_X = kS * (t_zeta * zeta_x + t_eta * eta_x + t_xi * xi_x) +
+                      (fl.u * TAU_SGS_XX + fl.v * TAU_SGS_XY + fl.w * TAU_SGS_XZ);
+            H_SGS_Y = kS * (t_zeta * zeta_y + t_eta * eta_y + t_xi * xi_y) +
+                      (fl.u * TAU_SGS_XY + fl.v * TAU_SGS_YY + fl.w * TAU_SGS_YZ);
+            H_SGS_Z = kS * (t_zeta * zeta_z + t_eta * eta_z + t_xi * xi_z) +
+                      (fl.u * TAU_SGS_XZ + fl.v * TAU_SGS_YZ + fl.w * TAU_SGS_ZZ);
+            D_SGS_X = D_SGS_Y = D_SGS_Z = 0.0;
+        }
 
         /* Viscous.cl:560-595 — Cartesian inviscid and viscous fluxes */
         const double rho = fl.rho, u = fl.u, v = fl.v, w = fl.w, p = fl.p;

@@ -42,6 +42,10 @@ typedef struct {
     double c_mts;     /* Viscous.cl:490 */
     double c_t;       /* Viscous.cl:491 */
     double eps;       /* Weno.cl:59 */
+    int32_t sgs;      /* 0: the reference's mixed-time-scale model; 1: Smagorinsky, nu_t = (c_s det^(1/3))^2 |S|
+                         (no reference counterpart; the reference kernels compiled as C ignore it) */
+    int32_t pad_;
+    double c_s;
 } orc_params;
 
 /* U_Q.cl:36-125 for Gid = 0 .. n_launch-1 */

@@ -28,13 +28,17 @@ assert FLOW_DT.itemsize == 72 and GRID_DT.itemsize == 56
 
 class Params(C.Structure):
     _fields_ = [("mach", C.c_double), ("reyl", C.c_double), ("pr", C.c_double), ("free_t", C.c_double),
-                ("c_mts", C.c_double), ("c_t", C.c_double), ("eps", C.c_double)]
+                ("c_mts", C.c_double), ("c_t", C.c_double), ("eps", C.c_double), ("sgs", C.c_int32),
+                ("pad_", C.c_int32), ("c_s", C.c_double)]
 
 
 def reference_params(**kw) -> Params:
     """The constants hard-coded in Viscous.cl:30-33,490-491 and Weno.cl:59."""
-    p = Params(1.5, 500.0, 0.72, 273.0, 0.03, 10.0, 10e-8)
+    p = Params(1.5, 500.0, 0.72, 273.0, 0.03, 10.0, 10e-8, 0, 0, 0.17)
+    names = {f[0] for f in Params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"reference_params: no parameter {k!r}")
         setattr(p, k, v)
     return p
 

@@ -5,6 +5,7 @@
 #include "cfdx.h"
 
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -26,8 +27,9 @@ int main(int argc, char **argv)
     cfdx_params p;
     cfdx_params_default(&p);
     CHECK(p.mach == 1.5 && p.reyl == 500.0 && p.pr == 0.72 && p.free_t == 273.0 && p.c_mts == 0.03 && p.c_t == 10.0 &&
-              p.eps == 1e-7 && p.precision == CFDX_FP64 && p.viscous == 1,
+              p.eps == 1e-7 && p.precision == CFDX_FP64 && p.viscous == 1 && p.sgs == CFDX_SGS_MTS && p.c_s == 0.17,
           "params_default");
+    CHECK(sizeof(cfdx_params) == 80 && offsetof(cfdx_params, c_s) == 72, "cfdx_params layout");
     printf("params ok\n");
 
     /* layouts of functions.h:49-112 */

@@ -29,7 +29,8 @@ def test_every_declared_symbol_is_exported():
 
 def test_layouts():
     assert cfdx.FLOW_DT.itemsize == 72 and cfdx.GRID_DT.itemsize == 56
-    assert C.sizeof(cfdx.Params) == 7 * 8 + 2 * 4
+    # 7 doubles, precision / viscous / sgs / pad (int32), c_s
+    assert C.sizeof(cfdx.Params) == 7 * 8 + 4 * 4 + 8 and cfdx.Params.c_s.offset == 72
     assert C.sizeof(cfdx.BlockDesc) == 64 and cfdx.BlockDesc.aoa.offset == 56   # 13 int32, pad, double
     assert cfdx.GRID_DT.fields["N0"][1] == 24 and cfdx.GRID_DT.fields["loc"][1] == 52
 

@@ -133,3 +133,30 @@ def test_block_steps(ctx, orc, kind, init, visc):
     got = b.get_flow()
     err = cases.rel_err(cases.flow_matrix(got), cases.flow_matrix(ref_case["FLOW"][1:]))
     assert err.max() <= TOL, err
+
+
+@pytest.mark.parametrize("inviscid_stokes", [False, True], ids=["re500", "re-inf"])
+def test_smagorinsky_option(ctx, orc, inviscid_stokes):
+    """The Smagorinsky SGS option (no reference counterpart; the oracle's restatement is pinned by
+    a closed form on linear shear): block RHS and 5 steps against the oracle, 1e-12 relative; with
+    Re = inf only the SGS terms remain."""
+    import pyoracle
+    kw = dict(reyl=math.inf) if inviscid_stokes else {}
+    case = cases.periodic(12, 10, 8, kind="sinusoid", init="tgv", mach=0.3)
+    prm = cfdx.default_params(sgs="smagorinsky", c_s=0.17, **kw)
+    prm_o = pyoracle.reference_params(sgs=1, c_s=0.17, **kw)
+    L_ref = orc.rhs(case, prm_o)[1:]
+    b = cfdx.Block(12, 10, 8, ctx=ctx)
+    b.set_case(case)
+    L = b.rhs(prm).T
+    assert cases.rel_err(L, L_ref).max() <= TOL
+    # the option changes the RHS (the SGS part is not negligible at this resolution)
+    L_mts = orc.rhs(case, pyoracle.reference_params(**kw))[1:]
+    assert np.abs(L_ref - L_mts).max() > 1e-6 * np.abs(L_ref).max()
+    dt = 0.5 * cfdx.cfl_dt(case, 1.0)
+    ref_case = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in case.items()}
+    orc.steps(ref_case, 5, 0.3, dt, prm_o)
+    b.step(5, 0.3, dt, prm)
+    err = cases.rel_err(cases.flow_matrix(b.get_flow()), cases.flow_matrix(ref_case["FLOW"][1:]))
+    b.close()
+    assert err.max() <= TOL, err

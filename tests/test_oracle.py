@@ -203,3 +203,65 @@ def test_discrete_conservation(orc):
     case = cases.periodic(9, 8, 7, noise=0.1)
     L = orc.rhs(case, pyoracle.reference_params())[1:]
     assert (np.abs(L.sum(axis=0)) <= 1e-12 * np.abs(L).sum(axis=0)).all()
+
+
+def test_smagorinsky_closed_form_on_linear_shear(orc):
+    """The Smagorinsky option on u = g y (v = w = 0, uniform p and rho) on a Cartesian box of
+    spacing h with Re = inf: the 4th-order differences are exact away from the periodic jump, S_xy
+    = g / 2, |S| = g, and the SGS stress is -2 rho (C_s h)^2 |S| S_xy. The det-scaled contravariant
+    eta flux of x-momentum is then det / h * tau_xy = -rho C_s^2 h^4 g^2, the eta flux of energy
+    (the stress's work u tau_xy) -rho C_s^2 h^4 g^2 u, and the xi flux of energy 0 (no
+    temperature gradient); C_s = 0 gives no viscous flux at all."""
+    import cfdx
+    n, h, g, cs = 8, 0.5, 0.3, 0.17
+    case = cfdx.periodic_case(n, 2 * n, n, grid="box", length=(n * h, 2 * n * h, n * h), init="freestream", mach=0.3)
+    y = case["xyz"][:, 1] if case["xyz"].ndim == 2 else case["xyz"][1]
+    F = case["FLOW"]
+    rho = float(F["rho"][1])
+    F["u"][1:] = g * y.reshape(-1)[: F.shape[0] - 1]
+    F["v"][1:] = 0.0
+    F["w"][1:] = 0.0
+    N = case["Grid_P"].shape[0]
+    w = orc.new_work(N)
+    for c_s, sgs in ((cs, 1), (0.0, 1)):
+        prm = pyoracle.reference_params(mach=0.3, reyl=math.inf, sgs=sgs, c_s=c_s)
+        orc.uq(N, w["U0"], w["Q_C"], case["Grid_P"], case["det"], F, w["ROE"])
+        orc.viscous(N, w["U0"], w["F"], w["E"], w["G"], w["Fv"], w["Ev"], w["Gv"], case["Grid_P"], case["metric"],
+                    case["det"], F, prm)
+        # interior rows: the +-2 stencil along y stays clear of the periodic jump at j = 0 / 2n - 1
+        j = (np.arange(1, N) - 1) // n % (2 * n)
+        inner = 1 + np.nonzero((j >= 2) & (j <= 2 * n - 3))[0]
+        want = -rho * c_s ** 2 * h ** 4 * g ** 2
+        scale = rho * cs ** 2 * h ** 4 * g ** 2
+        assert np.allclose(w["Ev"][inner, 1], want, rtol=0, atol=1e-12 * scale), (c_s, w["Ev"][inner, 1][:4], want)
+        assert np.allclose(w["Ev"][inner, 4], want * F["u"][inner], rtol=0, atol=1e-12 * scale * g * 2 * n * h)
+        # the SGS heat flux multiplies the temperature gradient, zero up to rounding
+        assert np.abs(w["Fv"][inner, 4]).max() <= 1e-11 * scale * g * 2 * n * h
+
+
+def test_smagorinsky_heat_flux_closed_form(orc):
+    """The Smagorinsky option's SGS heat flux: u = g y with p linear in x (rho uniform, so the
+    internal energy e = p / (0.4 rho) has the gradient b along x) and Re = inf: the xi flux of
+    energy is det / h * (-1.4 / Pr) rho (C_s h)^2 |S| b = -(1.4 / Pr) rho C_s^2 h^4 g b."""
+    import cfdx
+    n, h, g, cs, pr = 8, 0.5, 0.3, 0.17, 0.72
+    case = cfdx.periodic_case(2 * n, 2 * n, n, grid="box", length=(2 * n * h, 2 * n * h, n * h), init="freestream",
+                              mach=0.3)
+    xyz = case["xyz"].reshape(-1, 3) if case["xyz"].shape[-1] == 3 else case["xyz"].reshape(3, -1).T
+    F = case["FLOW"]
+    rho, p0 = float(F["rho"][1]), float(F["p"][1])
+    m = F.shape[0] - 1
+    F["u"][1:] = g * xyz[:m, 1]
+    F["p"][1:] = p0 * (1.0 + 0.05 * xyz[:m, 0])
+    b = p0 * 0.05 / (0.4 * rho)
+    N = case["Grid_P"].shape[0]
+    w = orc.new_work(N)
+    prm = pyoracle.reference_params(mach=0.3, reyl=math.inf, pr=pr, sgs=1, c_s=cs)
+    orc.uq(N, w["U0"], w["Q_C"], case["Grid_P"], case["det"], F, w["ROE"])
+    orc.viscous(N, w["U0"], w["F"], w["E"], w["G"], w["Fv"], w["Ev"], w["Gv"], case["Grid_P"], case["metric"],
+                case["det"], F, prm)
+    idx = np.arange(1, N) - 1
+    i, j = idx % (2 * n), idx // (2 * n) % (2 * n)
+    inner = 1 + np.nonzero((i >= 2) & (i <= 2 * n - 3) & (j >= 2) & (j <= 2 * n - 3))[0]
+    want = -(1.4 / pr) * rho * cs ** 2 * h ** 4 * g * b
+    assert np.allclose(w["Fv"][inner, 4], want, rtol=1e-10, atol=0), (w["Fv"][inner, 4][:4], want)
