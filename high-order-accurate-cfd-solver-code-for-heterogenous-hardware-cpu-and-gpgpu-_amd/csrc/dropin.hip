@@ -247,6 +247,8 @@ extern "C" int cfdx_viscous(cfdx_ctx *ctx, const cfdx_vec5 *U_C, cfdx_vec5 *F_C,
     if (!U_C || !F_C || !E_C || !G_C || !Fv_C || !Ev_C || !Gv_C || !Grid_P || !metric || !det || !FLOW)
         return cfdx::fail(CFDX_E_INVALID, "cfdx_viscous: null buffer");
     if (n_launch < 0 || n_launch > INT32_MAX) return cfdx::fail(CFDX_E_INVALID, "cfdx_viscous: n_launch out of range");
+    if (params && params->sgs != CFDX_SGS_MTS && params->sgs != CFDX_SGS_SMAGORINSKY)
+        return cfdx::fail(CFDX_E_INVALID, "cfdx_viscous: sgs %d is neither CFDX_SGS_MTS nor CFDX_SGS_SMAGORINSKY", params->sgs);
     if (n_launch == 0) return CFDX_OK;
     hipLaunchKernelGGL(k_viscous, dim3(cfdx::blocks_for(n_launch, 128)), dim3(128), 0, ctx->stream, (const double *)U_C,
                        (double *)F_C, (double *)E_C, (double *)G_C, (double *)Fv_C, (double *)Ev_C, (double *)Gv_C,
