@@ -123,3 +123,21 @@ def test_nonfinite_check(ctx):
         b.step(1, 0.3, dt, prm)
     assert e.value.code == cfdx.E_NONFINITE and "after step 3" in str(e.value)
     b.close()
+
+
+def test_params_refusals(ctx):
+    """Steps refuse an unknown SGS switch, a negative Smagorinsky constant and the Euler switch
+    while a model is on (CFDX_E_INVALID, message, nothing launched)."""
+    b = tgv(ctx)
+    dt = b.cfl_dt(0.5)
+    bad = [cfdx.default_params(sgs=7), cfdx.default_params(sgs="smagorinsky", c_s=-0.1),
+           cfdx.default_params(reyl=math.inf, c_mts=0.0, viscous=0, sgs="smagorinsky", c_s=0.17)]
+    for prm in bad:
+        with pytest.raises(cfdx.CfdxError) as e:
+            b.step(1, 0.3, dt, prm)
+        assert e.value.code == cfdx.E_INVALID
+    # the Euler switch with the Smagorinsky model off is accepted
+    b.step(1, 0.3, dt, cfdx.default_params(reyl=math.inf, viscous=0, sgs="smagorinsky", c_s=0.0))
+    b.close()
+    with pytest.raises(TypeError):
+        cfdx.default_params(cs=0.2)
