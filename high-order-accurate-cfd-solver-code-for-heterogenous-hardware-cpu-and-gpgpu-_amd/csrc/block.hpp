@@ -60,17 +60,23 @@ struct BlockDev {
     cfdx::dev::ViscParams vp;
 };
 
-// Nodes of the shell of a bounded block (within 3 nodes of a side along a bounded direction; the
-// node kernels leave them to the shell kernels); 0 for a periodic block.
+// Thickness of the shell of a bounded block: the loc-selected derivative forms differ from the
+// central ones only at a side's boundary node (every direction) and at the node next to it (the
+// form that reaches the ghost layers); two nodes from the side the stencil's outer points are the
+// boundary node and the interior, which select the central form.
+constexpr uint32_t SHELL = 2;
+
+// Nodes of the shell of a bounded block (within SHELL nodes of a side along a bounded direction;
+// the node kernels leave them to the shell kernels); 0 for a periodic block.
 __host__ __device__ inline uint32_t shell_count(const BlockDev &B)
 {
     if (!B.code) return 0;
     const uint32_t ni = B.ni, nj = B.nj, nk = B.nk;
-    const uint32_t kb = B.off[2] ? 3u : 0u, jb = B.off[1] ? 3u : 0u;
+    const uint32_t kb = B.off[2] ? SHELL : 0u, jb = B.off[1] ? SHELL : 0u;
     uint32_t n = 0;
-    if (B.off[2]) n += 6u * ni * nj;
-    if (B.off[1]) n += 6u * ni * (nk - 2 * kb);
-    if (B.off[0]) n += 6u * (nj - 2 * jb) * (nk - 2 * kb);
+    if (B.off[2]) n += 2 * SHELL * ni * nj;
+    if (B.off[1]) n += 2 * SHELL * ni * (nk - 2 * kb);
+    if (B.off[0]) n += 2 * SHELL * (nj - 2 * jb) * (nk - 2 * kb);
     return n;
 }
 

@@ -90,11 +90,11 @@ __device__ __forceinline__ bool near_side(const BlockDev &B, int i, int j, int k
     bool s = false;
 #pragma unroll
     for (int q = 0; q < 3; ++q) s |= B.off[q] && (c[q] == 0 || c[q] == n[q] - 1);
-    return s || (B.off[d] && (c[d] < 3 || c[d] > n[d] - 4));
+    return s || (B.off[d] && (c[d] < (int)SHELL || c[d] > n[d] - 1 - (int)SHELL));
 }
 
-// The shell of a bounded block: the nodes within 3 of a side along a bounded direction (the only
-// nodes where a loc-selected form can differ from the central one). The node kernels skip it and
+// The shell of a bounded block: the nodes within SHELL of a side along a bounded direction (the
+// only nodes where a loc-selected form can differ from the central one). The node kernels skip it and
 // the shell kernels cover it, so the tiled kernels carry no loc branches. Enumeration: the k shell
 // planes (if k is bounded), then the j shell rows inside the k range left, then the i shell
 // columns inside the j and k ranges left; i runs fastest in the first two parts.
@@ -103,17 +103,17 @@ __device__ __forceinline__ bool in_shell(const BlockDev &B, int i, int j, int k)
     const int c[3] = {i, j, k}, n[3] = {B.ni, B.nj, B.nk};
     bool s = false;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) s |= B.off[q] && (c[q] < 3 || c[q] > n[q] - 4);
+    for (int q = 0; q < 3; ++q) s |= B.off[q] && (c[q] < (int)SHELL || c[q] > n[q] - 1 - (int)SHELL);
     return s;
 }
 
 __device__ __forceinline__ void shell_node(const BlockDev &B, uint32_t t, int &i, int &j, int &k)
 {
     const uint32_t ni = B.ni, nj = B.nj, nk = B.nk;
-    const uint32_t kb = B.off[2] ? 3u : 0u, jb = B.off[1] ? 3u : 0u;
-    auto layer = [](uint32_t m, uint32_t n) { return (int)(m < 3 ? m : n - 6 + m); };
+    const uint32_t kb = B.off[2] ? SHELL : 0u, jb = B.off[1] ? SHELL : 0u, w = 2 * SHELL;
+    auto layer = [](uint32_t m, uint32_t n) { return (int)(m < SHELL ? m : n - 2 * SHELL + m); };
     if (B.off[2]) {
-        const uint32_t part = 6u * ni * nj;
+        const uint32_t part = w * ni * nj;
         if (t < part) {
             i = (int)(t % ni);
             j = (int)((t / ni) % nj);
@@ -123,18 +123,18 @@ __device__ __forceinline__ void shell_node(const BlockDev &B, uint32_t t, int &i
         t -= part;
     }
     if (B.off[1]) {
-        const uint32_t part = 6u * ni * (nk - 2 * kb);
+        const uint32_t part = w * ni * (nk - 2 * kb);
         if (t < part) {
             i = (int)(t % ni);
-            j = layer((t / ni) % 6u, nj);
-            k = (int)(kb + t / (6u * ni));
+            j = layer((t / ni) % w, nj);
+            k = (int)(kb + t / (w * ni));
             return;
         }
         t -= part;
     }
-    i = layer(t % 6u, ni);
-    j = (int)(jb + (t / 6u) % (nj - 2 * jb));
-    k = (int)(kb + t / (6u * (nj - 2 * jb)));
+    i = layer(t % w, ni);
+    j = (int)(jb + (t / w) % (nj - 2 * jb));
+    k = (int)(kb + t / (w * (nj - 2 * jb)));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), each
