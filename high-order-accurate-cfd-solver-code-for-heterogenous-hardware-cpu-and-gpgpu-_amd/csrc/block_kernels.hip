@@ -317,10 +317,11 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
             const int s = DIR == 0 ? p % NP : p / TA;
             const int a = DIR == 0 ? p / NP : p % TA;
             if (a0 + a >= n_across) continue;
+            // the points past the stencil of the tile's last computed face feed no face: skip them
+            // (a bounded line ends 3 ghost nodes past its last owned node; a periodic line shorter
+            // than the tile would wrap more than once)
+            if (s > nfa - b0 + 4) continue;
             const int along = along_shift(B, DIR, b0 - sh, s - 2, n_along);
-            // a bounded line ends 3 ghost nodes past the last owned node; the points of a partial
-            // last tile beyond it feed no computed face
-            if (sh && along > n_along + 2) continue;
             const uint32_t x = tile_node<DIR>(B, along, a0 + a, fixed);
             const S rho = ld(P0, x), u = ld(P1, x), v = ld(P2, x), w = ld(P3, x), pr = ld(P4, x);
             S q[5], f[5];

@@ -54,6 +54,9 @@ CASES = {
     "cgrid": cgrid_case,
     "box-walls-j": lambda: box_case(4, (3, 2, 1, 1, 0, 0), (14, 12, 6)),
     "box-all-bounded": lambda: box_case(0, (4, 4, 1, 2, 1, 3), (10, 9, 8)),
+    # the smallest bounded directions (7 nodes: the two-node shells leave 3 interior nodes) and
+    # a periodic k of 3
+    "box-minimal": lambda: box_case(4, (1, 3, 2, 1, 0, 0), (7, 7, 3)),
 }
 
 

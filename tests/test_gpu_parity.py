@@ -160,3 +160,26 @@ def test_smagorinsky_option(ctx, orc, inviscid_stokes):
     err = cases.rel_err(cases.flow_matrix(b.get_flow()), cases.flow_matrix(ref_case["FLOW"][1:]))
     b.close()
     assert err.max() <= TOL, err
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 3), (3, 3, 3), (40, 3, 5)])
+def test_block_steps_small_periodic(ctx, orc, shape):
+    """Periodic directions shorter than a face tile (the stencil of a line wraps around it): every
+    tile variant against the oracle over 3 steps, 1e-12 relative."""
+    import pyoracle
+    ni, nj, nk = shape
+    case = cfdx.periodic_case(ni, nj, nk, grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.2 * 2 * math.pi / max(shape),
+                              phases=(0.3, 1.1, 2.0), init="tgv", mach=0.3, noise=0.02)
+    dt = 0.5 * cfdx.cfl_dt(case, 1.0)
+    ref = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in case.items()}
+    orc.steps(ref, 3, 0.3, dt, pyoracle.reference_params())
+    want = cases.flow_matrix(ref["FLOW"][1:])
+    counts = [len(cfdx.tile_variants(k)) for k in (0, 1, 1, 2)]
+    for v in range(max(counts)):
+        b = cfdx.Block(ni, nj, nk, ctx=ctx)
+        b.set_tiles(tuple(min(v, c - 1) for c in counts))
+        b.set_case(case)
+        b.step(3, 0.3, dt, cfdx.default_params())
+        err = cases.rel_err(cases.flow_matrix(b.get_flow()), want)
+        b.close()
+        assert err.max() <= TOL, (shape, v, err)
