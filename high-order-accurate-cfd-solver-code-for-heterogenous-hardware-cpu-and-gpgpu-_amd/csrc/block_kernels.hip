@@ -292,8 +292,13 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 {
     static_assert(TA * TB == 256, "face tiles have 256 threads");
     constexpr int NP = TB + 5;
+    // mixed form: the Roe state's inputs (rho, u, v, w, h) of the stencil points are staged too, so
+    // phase 2 reads the two centre nodes from LDS (in FP64 the extra live range costs a wave per
+    // SIMD, so it loads them from global memory instead)
+    constexpr bool STAGE_ROE = std::is_same<S, float>::value;
     __shared__ S sq[5 * NP * TA];
     __shared__ S sf[5 * NP * TA];
+    __shared__ S sp[STAGE_ROE ? 5 * NP * TA : 1];
     const int tid = threadIdx.x;
     for (uint32_t wg = blockIdx.x; wg < total; wg += gridDim.x) {
         const uint32_t t = xcd_tile(wg, total);
@@ -333,6 +338,11 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
                 sq[lds_at<DIR, TA, TB>(c, s, a)] = q[c];
                 sf[lds_at<DIR, TA, TB>(c, s, a)] = f[c];
             }
+            if constexpr (STAGE_ROE) {
+                sp[lds_at<DIR, TA, TB>(0, s, a)] = rho; sp[lds_at<DIR, TA, TB>(1, s, a)] = u;
+                sp[lds_at<DIR, TA, TB>(2, s, a)] = v; sp[lds_at<DIR, TA, TB>(3, s, a)] = w;
+                sp[lds_at<DIR, TA, TB>(4, s, a)] = enthalpy(rho, u, v, w, pr);
+            }
         }
         __syncthreads();
 
@@ -345,9 +355,13 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 #pragma unroll
                 for (int c = 0; c < 5; ++c) q[m][c] = sq[lds_at<DIR, TA, TB>(c, b + m, a)];
             const uint32_t gl = tile_node<DIR>(B, b0 + b - sh, a0 + a, fixed);
-            const uint32_t gr = tile_node<DIR>(B, along_shift(B, DIR, b0 + b - sh, 1, n_along), a0 + a, fixed);
             S roe[5];
-            {
+            if constexpr (STAGE_ROE) {
+                // the face's two centre nodes are stencil points 2 and 3
+                auto at = [&](int c, int m) { return sp[lds_at<DIR, TA, TB>(c, b + m, a)]; };
+                roe_average(at(0, 2), at(1, 2), at(2, 2), at(3, 2), at(4, 2), at(0, 3), at(1, 3), at(2, 3), at(3, 3), at(4, 3), roe);
+            } else {
+                const uint32_t gr = tile_node<DIR>(B, along_shift(B, DIR, b0 + b - sh, 1, n_along), a0 + a, fixed);
                 const S ra = ld(P0, gl), ua = ld(P1, gl), va = ld(P2, gl), wa = ld(P3, gl), pa = ld(P4, gl);
                 const S rb = ld(P0, gr), ub = ld(P1, gr), vb = ld(P2, gr), wb = ld(P3, gr), pb = ld(P4, gr);
                 roe_average(ra, ua, va, wa, enthalpy(ra, ua, va, wa, pa), rb, ub, vb, wb, enthalpy(rb, ub, vb, wb, pb), roe);
