@@ -103,16 +103,22 @@ def make_block(req: Request, precision: int, ctx: Ctx, prep: dict | None = None)
         prep = prep or prepare(req)
         d = BlockDesc.from_buffer_copy(prep["desc"])
         b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx, desc=d)
-        b.set_geometry_soa(prep["geo"])
-        b.configure(req.mach, 0.0, req.params())      # the side conditions use the block's Mach
-        b.set_flow(prep["flow"])
     else:
         b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx)
-        amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
-        b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
-                    beta=req.beta, center=req.center)
-    dt = b.cfl_dt(req.cfl)
-    b.configure(req.mach, dt, req.params())
+    try:
+        if req.grid == "cgrid":
+            b.set_geometry_soa(prep["geo"])
+            b.configure(req.mach, 0.0, req.params())      # the side conditions use the block's Mach
+            b.set_flow(prep["flow"])
+        else:
+            amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
+            b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
+                        beta=req.beta, center=req.center)
+        dt = b.cfl_dt(req.cfl)
+        b.configure(req.mach, dt, req.params())
+    except BaseException:
+        b.close()
+        raise
     return b, dt
 
 
@@ -194,86 +200,108 @@ class Server:
             return sum(a["req"].nodes for a in active.values()) + sum(a["req"].nodes for a in admitted)
 
         preps: dict[int, concurrent.futures.Future] = {}
-        while pending or queue or active or admitted or inflight:
-            t = now()
-            while pending and pending[0].arrival <= t:
-                req = pending.popleft()
-                if order is None:
-                    queue.append(req)
-                else:
-                    bisect.insort(queue, req, key=order)
-                if req.grid == "cgrid":
-                    preps[req.rid] = self.pool.submit(prepare, req)
-            # admission in queue order while memory, slots and the node cap allow, once the host set-up is
-            # ready (waited for when nothing else could run); the block is built on the set-up
-            # stream while a chunk may be running
-            busy = bool(active or admitted)
-            while queue and len(active) + len(admitted) < self.max_active and self._fits(
-                    queue[0], used, nodes_in_flight(), busy):
-                req = queue[0]
-                fut = preps.get(req.rid)
-                if fut is not None and not fut.done() and (busy or inflight):
-                    break
-                queue.pop(0)
-                prep = preps.pop(req.rid).result() if fut is not None else None
-                b, dt = make_block(req, self.precision, self.io, prep)
-                nb = req.bytes(self.precision)
-                used += nb
-                admitted.append(dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb))
-                busy = True
-            if queue and not (active or admitted or inflight):
-                raise RuntimeError(f"request {queue[0].rid} ({queue[0].ni}x{queue[0].nj}x{queue[0].nk}) does not fit "
-                                   "in the free device memory")
-            # the chunk in flight completes
-            done = []
-            if inflight:
-                self.ctx.sync()
-                self.chunks += 1
-                vclock += inflight
-                for rid, a in active.items():
-                    a["left"] -= inflight
-                    if a["left"] == 0:
-                        done.append(active[rid])
-                inflight = 0
-                for b in retired:
-                    b.close()
-                retired = []
-            for a in done:
-                active.pop(a["req"].rid)
-                self.batch.remove(a["block"])
-            for a in admitted:
-                self.batch.add(a["block"])
-                active[a["req"].rid] = a
-            admitted = []
-            # next chunk, asynchronous
-            if active:
-                self.peak_active = max(self.peak_active, len(active))
-                self.peak_bytes = max(self.peak_bytes, used)
-                k = min(self.chunk, min(a["left"] for a in active.values()))
-                if clock == "steps" and pending:
-                    k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
-                self.batch.step(k, self.graph)
-                inflight = k
-                if not self.pipelined:
+        try:
+            while pending or queue or active or admitted or inflight:
+                t = now()
+                while pending and pending[0].arrival <= t:
+                    req = pending.popleft()
+                    if order is None:
+                        queue.append(req)
+                    else:
+                        bisect.insort(queue, req, key=order)
+                    if req.grid == "cgrid":
+                        preps[req.rid] = self.pool.submit(prepare, req)
+                # admission in queue order while memory, slots and the node cap allow, once the host set-up is
+                # ready (waited for when nothing else could run); the block is built on the set-up
+                # stream while a chunk may be running
+                busy = bool(active or admitted)
+                while queue and len(active) + len(admitted) < self.max_active and self._fits(
+                        queue[0], used, nodes_in_flight(), busy):
+                    req = queue[0]
+                    fut = preps.get(req.rid)
+                    if fut is not None and not fut.done() and (busy or inflight):
+                        break
+                    queue.pop(0)
+                    prep = preps.pop(req.rid).result() if fut is not None else None
+                    b, dt = make_block(req, self.precision, self.io, prep)
+                    nb = req.bytes(self.precision)
+                    used += nb
+                    admitted.append(dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb))
+                    busy = True
+                if queue and not (active or admitted or inflight):
+                    raise RuntimeError(f"request {queue[0].rid} ({queue[0].ni}x{queue[0].nj}x{queue[0].nk}) does not fit "
+                                       "in the free device memory")
+                # the chunk in flight completes
+                done = []
+                if inflight:
                     self.ctx.sync()
-            # results of the requests that finished, copied on the set-up stream during the chunk
-            for a in done:
-                b = a["block"]
-                flow = b.get_flow() if self.keep_flow else None
-                used -= a["bytes"]
-                results[a["req"].rid] = Result(a["req"].rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow)
-                retired.append(b)
-            if not (active or admitted or inflight) and pending and not queue:
-                for b in retired:
-                    b.close()
-                retired = []
-                if clock == "steps":   # idle until the next arrival
-                    vclock = max(vclock, int(math.ceil(pending[0].arrival)))
-                else:
-                    time.sleep(max(0.0, pending[0].arrival - now()))
-        for b in retired:
-            b.close()
-        return results
+                    self.chunks += 1
+                    vclock += inflight
+                    for rid, a in active.items():
+                        a["left"] -= inflight
+                        if a["left"] == 0:
+                            done.append(active[rid])
+                    inflight = 0
+                    for b in retired:
+                        b.close()
+                    retired = []
+                for a in done:
+                    active.pop(a["req"].rid)
+                    self.batch.remove(a["block"])
+                for a in admitted:
+                    self.batch.add(a["block"])
+                    active[a["req"].rid] = a
+                admitted = []
+                # next chunk, asynchronous
+                if active:
+                    self.peak_active = max(self.peak_active, len(active))
+                    self.peak_bytes = max(self.peak_bytes, used)
+                    k = min(self.chunk, min(a["left"] for a in active.values()))
+                    if clock == "steps" and pending:
+                        k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
+                    self.batch.step(k, self.graph)
+                    inflight = k
+                    if not self.pipelined:
+                        self.ctx.sync()
+                # results of the requests that finished, copied on the set-up stream during the chunk
+                for a in done:
+                    b = a["block"]
+                    flow = b.get_flow() if self.keep_flow else None
+                    used -= a["bytes"]
+                    results[a["req"].rid] = Result(a["req"].rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow)
+                    retired.append(b)
+                if not (active or admitted or inflight) and pending and not queue:
+                    for b in retired:
+                        b.close()
+                    retired = []
+                    if clock == "steps":   # idle until the next arrival
+                        vclock = max(vclock, int(math.ceil(pending[0].arrival)))
+                    else:
+                        time.sleep(max(0.0, pending[0].arrival - now()))
+            for b in retired:
+                b.close()
+            return results
+        except BaseException:
+            self._abandon(active, admitted, retired)
+            raise
+
+    def _abandon(self, active: dict, admitted: list, retired: list) -> None:
+        """After a failure inside run(): waits for the device and releases every block the run
+        holds, so that the server and its batch can be used again."""
+        try:
+            self.ctx.sync()
+        except Exception:
+            pass
+        for a in list(active.values()):
+            try:
+                self.batch.remove(a["block"])
+            except Exception:
+                pass
+        for b in [a["block"] for a in active.values()] + [a["block"] for a in admitted] + list(retired):
+            try:
+                b.close()
+            except Exception:
+                pass
 
     def close(self) -> None:
         self.batch.close()

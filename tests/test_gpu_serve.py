@@ -120,6 +120,26 @@ def test_serving_sjf_admits_least_work_first(ctx):
         serve.Server(precision=cfdx.FP64, ctx=ctx, policy="lifo")
 
 
+def test_serving_failure_releases_blocks(ctx):
+    """A request that fails in its set-up while another is on the GPU: run() raises, every block
+    of the run is released and the batch is empty, and the same server then serves a valid
+    stream with the results of each request alone."""
+    reqs = small_stream()[:2]
+    reqs[1].arrival = 2
+    reqs[1].init = "no-such-state"
+    srv = serve.Server(precision=cfdx.FP64, ctx=ctx, reserve_bytes=0, chunk=2)
+    with pytest.raises(Exception):
+        srv.run(reqs, clock="steps")
+    assert len(srv.batch) == 0
+    good = small_stream()[2:4]
+    res = srv.run(good, clock="steps")
+    srv.close()
+    for r in good:
+        alone = serve.run_alone(r, cfdx.FP64, ctx)
+        for f in FIELDS:
+            assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
+
+
 def test_serving_wall_clock(ctx):
     reqs = small_stream()
     for r in reqs:
