@@ -36,6 +36,7 @@ struct cfdx_block {
     double *mon = nullptr;        // residual monitor partials (3 doubles per wave) + 3 reduced values
     uint32_t mon_cap = 0;         // partial triples allocated
     uint32_t mon_count = 0;       // partial triples written by the last step
+    std::vector<cfdx_batch *> batches;   // batches holding the block (it leaves them when destroyed)
 };
 
 struct cfdx_batch {
@@ -441,6 +442,14 @@ extern "C" int cfdx_block_destroy(cfdx_block *b)
     if (!b) return CFDX_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
+    // a destroyed block leaves the batches that hold it, after their work on it has finished
+    for (cfdx_batch *q : b->batches) {
+        (void)hipStreamSynchronize(q->ctx->stream);
+        auto it = std::find(q->blocks.begin(), q->blocks.end(), b);
+        if (it != q->blocks.end()) q->blocks.erase(it);
+        q->dirty = true;
+    }
+    b->batches.clear();
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (b->mem) cfdx::pool_give(b->ctx->device, b->mem, b->alloc);   // dB and dprefix live at its end
@@ -801,6 +810,10 @@ extern "C" int cfdx_batch_destroy(cfdx_batch *q)
     if (!q) return CFDX_OK;
     (void)hipSetDevice(q->ctx->device);
     (void)hipStreamSynchronize(q->ctx->stream);
+    for (cfdx_block *b : q->blocks) {
+        auto ib = std::find(b->batches.begin(), b->batches.end(), q);
+        if (ib != b->batches.end()) b->batches.erase(ib);
+    }
     drop_graph(q);
     if (q->dB) (void)hipFree(q->dB);
     if (q->dprefix) (void)hipFree(q->dprefix);
@@ -817,6 +830,7 @@ extern "C" int cfdx_batch_add(cfdx_batch *q, cfdx_block *b)
     if (!b->configured) return cfdx::fail(CFDX_E_INVALID, "batch_add: configure the block (cfdx_block_configure) first");
     if (std::find(q->blocks.begin(), q->blocks.end(), b) != q->blocks.end()) return cfdx::fail(CFDX_E_INVALID, "batch_add: block already in the batch");
     q->blocks.push_back(b);
+    b->batches.push_back(q);
     q->dirty = true;
     return CFDX_OK;
 }
@@ -827,6 +841,8 @@ extern "C" int cfdx_batch_remove(cfdx_batch *q, cfdx_block *b)
     auto it = std::find(q->blocks.begin(), q->blocks.end(), b);
     if (it == q->blocks.end()) return cfdx::fail(CFDX_E_INVALID, "batch_remove: block not in the batch");
     q->blocks.erase(it);
+    auto ib = std::find(b->batches.begin(), b->batches.end(), q);
+    if (ib != b->batches.end()) b->batches.erase(ib);
     q->dirty = true;
     return CFDX_OK;
 }

@@ -293,6 +293,8 @@ typedef struct cfdx_batch cfdx_batch;
 int cfdx_batch_create(cfdx_ctx *ctx, int precision, cfdx_batch **out);
 int cfdx_batch_destroy(cfdx_batch *q);
 int cfdx_batch_add(cfdx_batch *q, cfdx_block *b);      /* the block must be configured */
+/* A block leaves a batch by cfdx_batch_remove, or by cfdx_block_destroy (which waits for the
+ * batch's stream first); cfdx_batch_destroy leaves its blocks alive. */
 int cfdx_batch_remove(cfdx_batch *q, cfdx_block *b);
 int cfdx_batch_size(cfdx_batch *q);
 /* grid_cap = 0: one workgroup per tile; > 0: persistent launches of at most grid_cap workgroups

@@ -157,3 +157,30 @@ def test_overlap_gives_the_same_bits(ctx, graph):
         for f in ("rho", "u", "v", "w", "p"):
             assert np.array_equal(a[f], r[f]), f
             assert np.isfinite(a[f]).all()
+
+
+def test_destroyed_block_leaves_its_batches(ctx):
+    """A block destroyed while a batch (captured graph) holds it leaves the batch: the batch then
+    steps the remaining block, which gets its result alone."""
+    prm = cfdx.default_params(mach=0.2, reyl=900.0)
+    q = cfdx.Batch(cfdx.MIXED, ctx)
+    a, b = ragged(ctx, cfdx.MIXED, 24, 16, 8), ragged(ctx, cfdx.MIXED, 19, 13, 11)
+    for x in (a, b):
+        x.configure(0.2, x.cfl_dt(0.5), prm)
+        q.add(x)
+    q.step(2, True)
+    ctx.sync()
+    b.close()                      # not removed first
+    assert len(q) == 1
+    q.step(2, True)
+    ctx.sync()
+    got = a.get_flow()
+    q.close()
+    alone = ragged(ctx, cfdx.MIXED, 24, 16, 8)
+    alone.configure(0.2, alone.cfl_dt(0.5), prm)
+    alone.step(4, 0.2, alone.cfl_dt(0.5), prm)
+    want = alone.get_flow()
+    for f in ("rho", "u", "v", "w", "p"):
+        assert np.array_equal(got[f], want[f]), f
+    a.close()
+    alone.close()
