@@ -39,6 +39,28 @@ __device__ __forceinline__ void st(T *base, uint32_t i, T v)
     *(CFDX_GLOBAL T *)(p + i * (uint32_t)sizeof(T)) = v;
 }
 
+// Streaming forms for data a kernel touches once (geometry read at its own node, outputs): the
+// non-temporal hint lets the L2 evict these lines first, keeping the stencil neighbours that
+// adjacent tiles read again. Used for periodic blocks only (measured at 128^3: the whole step
+// 5 % faster in FP64 and mixed, 64 cubes 4 %); on the bounded C-grid (384x128x32) the hint
+// costs 1 % in FP64 and 4 % mixed, so lists with bounded blocks keep plain accesses.
+template <bool NT, typename T>
+__device__ __forceinline__ T ldx(const T *base, uint32_t i)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load((const CFDX_GLOBAL T *)((const CFDX_GLOBAL char *)base + i * (uint32_t)sizeof(T)));
+    else
+        return ld(base, i);
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void stx(T *base, uint32_t i, T v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, (CFDX_GLOBAL T *)((CFDX_GLOBAL char *)base + i * (uint32_t)sizeof(T)));
+    else
+        st(base, i, v);
+}
+
 // Geometry array a (0 det, 1-9 node metrics, 10-18 face metric rows, 19-21 face dets) in the flux
 // type: the double arrays in an FP64 block, their float copies in a mixed block (the flux
 // arithmetic reads the metrics in S; the copies hold the same roundings a conversion at the load
@@ -162,7 +184,7 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ pref
 // ---------------------------------------------------------------------------
 // Fv, Ev, Gv (+ mu) of one node. LOC: a shell node of a bounded block (loc-selected forms near
 // the sides); otherwise the central forms.
-template <typename S, bool LOC>
+template <typename S, bool LOC, bool NT>
 __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k, int mode)
 {
     const uint32_t g = node_index(B, i, j, k);
@@ -215,7 +237,7 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
 #pragma unroll
     for (int d = 0; d < 3; ++d) KSs[d] = S(KS[d]);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) m[c] = ld(geo<S>(B, 1 + c), g);
+    for (int c = 0; c < 9; ++c) m[c] = ldx<NT>(geo<S>(B, 1 + c), g);
     S fv[3][5];
     // the node velocity: away from the sides, from the stage vector already in registers (U_m /
     // U_0, the primitives of P up to rounding) instead of three more loads; near the sides the
@@ -224,12 +246,12 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
 #pragma unroll
     for (int c = 0; c < 3; ++c) vel[c] = LOC ? ld((const S *)B.P[1 + c], g) : S(U[1 + c] * iU0);
     const S mu = viscous_node(dUs, Uv, vel[0], vel[1], vel[2], m,
-                              ld(geo<S>(B, 0), g), KSs, B.vp, fv);
+                              ldx<NT>(geo<S>(B, 0), g), KSs, B.vp, fv);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int c = 1; c < 5; ++c) st((S *)B.Fv[d][c - 1], g, fv[d][c]);
-    if (mode >= 2) st(B.mu, g, (double)mu);
+        for (int c = 1; c < 5; ++c) stx<NT>((S *)B.Fv[d][c - 1], g, fv[d][c]);
+    if (mode >= 2) stx<NT>(B.mu, g, (double)mu);
 }
 
 // Node tiles of the list first (tile ids through the XCD remap), then, for lists with bounded
@@ -248,7 +270,7 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
             if (!B.viscous || t >= shell_count(B)) continue;
             int i, j, k;
             shell_node(B, t, i, j, k);
-            visc_node<S, true>(B, i, j, k, mode);
+            visc_node<S, true, !BND>(B, i, j, k, mode);
             continue;
         }
         const uint32_t t = xcd_tile(wg, total);
@@ -261,7 +283,7 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
         const int k = (int)(lt / (ntx * nty));
         if (i >= B.ni || j >= B.nj) continue;
         if (BND && B.code && in_shell(B, i, j, k)) continue;   // a shell tile's node
-        visc_node<S, false>(B, i, j, k, mode);
+        visc_node<S, false, !BND>(B, i, j, k, mode);
     }
 }
 
@@ -366,15 +388,32 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
                 const S rb = ld(P0, gr), ub = ld(P1, gr), vb = ld(P2, gr), wb = ld(P3, gr), pb = ld(P4, gr);
                 roe_average(ra, ua, va, wa, enthalpy(ra, ua, va, wa, pa), rb, ub, vb, wb, enthalpy(rb, ub, vb, wb, pb), roe);
             }
+            // face metric row and face det; streaming accesses on periodic blocks (B.code null)
+            S fg[4];
+            if (B.code) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) fg[c] = ld(geo<S>(B, 10 + 3 * DIR + c), gl);
+                fg[3] = ld(geo<S>(B, 19 + DIR), gl);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) fg[c] = ldx<true>(geo<S>(B, 10 + 3 * DIR + c), gl);
+                fg[3] = ldx<true>(geo<S>(B, 19 + DIR), gl);
+            }
             S llf[5];
-            face_llf(q, roe, ld(geo<S>(B, 10 + 3 * DIR), gl), ld(geo<S>(B, 11 + 3 * DIR), gl), ld(geo<S>(B, 12 + 3 * DIR), gl),
-                     ld(geo<S>(B, 19 + DIR), gl), S(B.eps), llf);
+            face_llf(q, roe, fg[0], fg[1], fg[2], fg[3], S(B.eps), llf);
+            S fh[5];
 #pragma unroll
             for (int c = 0; c < 5; ++c)
-                st((S *)B.FH[DIR][c], gl,
-                   S(llf[c] + midpoint_correction(sf[lds_at<DIR, TA, TB>(c, b, a)], sf[lds_at<DIR, TA, TB>(c, b + 1, a)],
-                                                  sf[lds_at<DIR, TA, TB>(c, b + 2, a)], sf[lds_at<DIR, TA, TB>(c, b + 3, a)],
-                                                  sf[lds_at<DIR, TA, TB>(c, b + 4, a)], sf[lds_at<DIR, TA, TB>(c, b + 5, a)])));
+                fh[c] = S(llf[c] + midpoint_correction(sf[lds_at<DIR, TA, TB>(c, b, a)], sf[lds_at<DIR, TA, TB>(c, b + 1, a)],
+                                                       sf[lds_at<DIR, TA, TB>(c, b + 2, a)], sf[lds_at<DIR, TA, TB>(c, b + 3, a)],
+                                                       sf[lds_at<DIR, TA, TB>(c, b + 4, a)], sf[lds_at<DIR, TA, TB>(c, b + 5, a)]));
+            if (B.code) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) st((S *)B.FH[DIR][c], gl, fh[c]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) stx<true>((S *)B.FH[DIR][c], gl, fh[c]);
+            }
         }
     }
 }
@@ -385,7 +424,7 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
 // ---------------------------------------------------------------------------
 // L and the SSP-RK3 stage of one node; mdu / mdv / mde: the node's monitor values (mode 2 with
 // the monitor on). LOC as in visc_node.
-template <typename S, bool LOC>
+template <typename S, bool LOC, bool NT>
 __device__ __forceinline__ void update_node(const BlockDev &B, int i, int j, int k, int mode, double &mdu, double &mdv,
                                             double &mde)
 {
@@ -435,30 +474,30 @@ __device__ __forceinline__ void update_node(const BlockDev &B, int i, int j, int
         const double del_t = B.del_t;
         double un[5], u0[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) u0[c] = ld(B.U0[c], g);
+        for (int c = 0; c < 5; ++c) u0[c] = ldx<NT>(B.U0[c], g);
         if (mode == 0) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) { un[c] = u0[c] + del_t * L[c]; st(B.Us[c], g, un[c]); }
+            for (int c = 0; c < 5; ++c) { un[c] = u0[c] + del_t * L[c]; stx<NT>(B.Us[c], g, un[c]); }
         } else if (mode == 1) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) { un[c] = 0.75 * u0[c] + 0.25 * ld(B.Us[c], g) + 0.25 * del_t * L[c]; st(B.Us[c], g, un[c]); }
+            for (int c = 0; c < 5; ++c) { un[c] = 0.75 * u0[c] + 0.25 * ldx<NT>(B.Us[c], g) + 0.25 * del_t * L[c]; stx<NT>(B.Us[c], g, un[c]); }
         } else {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) un[c] = (u0[c] * (1.0 / 3.0)) + ((2.0 / 3.0) * ld(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
+            for (int c = 0; c < 5; ++c) un[c] = (u0[c] * (1.0 / 3.0)) + ((2.0 / 3.0) * ldx<NT>(B.Us[c], g)) + (2.0 / 3.0) * (del_t * L[c]);
         }
-        const double det = ld(B.det, g);
+        const double det = ldx<NT>(B.det, g);
         const double idet = 1.0 / det;
         const double rho = idet * un[0], ir = 1.0 / rho;
         const double u = (idet * un[1]) * ir, v = (idet * un[2]) * ir, w = (idet * un[3]) * ir;
         const double e = (idet * un[4]) * ir - 0.5 * (u * u + v * v + w * w);
         const double p = e * rho * 0.4;
-        st((S *)B.P[0], g, S(rho)); st((S *)B.P[1], g, S(u)); st((S *)B.P[2], g, S(v)); st((S *)B.P[3], g, S(w)); st((S *)B.P[4], g, S(p));
+        stx<NT>((S *)B.P[0], g, S(rho)); stx<NT>((S *)B.P[1], g, S(u)); stx<NT>((S *)B.P[2], g, S(v)); stx<NT>((S *)B.P[3], g, S(w)); stx<NT>((S *)B.P[4], g, S(p));
         if (mode == 2) {
             // next step's RK base: U_Q recomputes det*Q from the new primitives (Solver.cpp:377-388)
             double q[5];
             prim_to_q(rho, u, v, w, p, q);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) st(B.U0[c], g, det * q[c]);
+            for (int c = 0; c < 5; ++c) stx<NT>(B.U0[c], g, det * q[c]);
             if (B.mon) {
                 const double r0 = idet * u0[0], ir0 = 1.0 / r0;
                 const double uo = (idet * u0[1]) * ir0, vo = (idet * u0[2]) * ir0, wo = (idet * u0[3]) * ir0;
@@ -503,7 +542,7 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
             if (t < ns) {
                 int i, j, k;
                 shell_node(B, t, i, j, k);
-                update_node<S, true>(B, i, j, k, mode, mdu, mdv, mde);
+                update_node<S, true, !BND>(B, i, j, k, mode, mdu, mdv, mde);
             }
             if (mode == 2 && B.mon && (t & ~63u) < ns) {
                 const uint32_t main_slots = ((B.ni + TX - 1) / TX) * ((B.nj + TY - 1) / TY) * B.nk * (uint32_t)(TX * TY / 64);
@@ -519,7 +558,7 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
         const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
         const int k = (int)(lt / (ntx * nty));
         if (i < B.ni && j < B.nj && !(BND && B.code && in_shell(B, i, j, k)))
-            update_node<S, false>(B, i, j, k, mode, mdu, mdv, mde);
+            update_node<S, false, !BND>(B, i, j, k, mode, mdu, mdv, mde);
         if (mode == 2 && B.mon) monitor_wave(B, lt * (TX * TY / 64) + threadIdx.x / 64, mdu, mdv, mde);
     }
 }
