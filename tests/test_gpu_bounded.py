@@ -113,20 +113,24 @@ def test_cgrid_mixed_vs_fp64(ctx):
     assert all(e <= 1e-5 for e in errs.values()), errs
 
 
-def test_batch_with_bounded_blocks(ctx):
+@pytest.mark.parametrize("precision", [cfdx.FP64, cfdx.MIXED])
+def test_batch_with_bounded_blocks(ctx, precision):
+    """Bounded and periodic blocks in one batch give the bits of each block run alone. The
+    periodic block alone runs the node kernels with non-temporal accesses, in the batch (a list
+    with bounded blocks) with plain ones: a cache hint, so the bits must not change."""
     prm = cfdx.default_params(mach=0.5, reyl=1e5)
     alone, blocks = [], []
     for d, xyz, geo, mach, aoa, pk in (cgrid_case(), box_case(0, (4, 4, 1, 2, 1, 3), (10, 9, 8))):
         own = cfdx.owned_index(d)
         flow = perturbed(xyz[own], mach, aoa)
         p = cfdx.default_params(**pk)
-        b, dt = run_block(ctx, d, geo, flow, mach, None, p, 3)
+        b, dt = run_block(ctx, d, geo, flow, mach, None, p, 3, precision)
         alone.append(b.get_flow())
-        b2, _ = run_block(ctx, d, geo, flow, mach, dt, p, 0)
+        b2, _ = run_block(ctx, d, geo, flow, mach, dt, p, 0, precision)
         blocks.append(b2)
 
     def tgv():
-        b = cfdx.Block(12, 10, 8, ctx=ctx)
+        b = cfdx.Block(12, 10, 8, precision=precision, ctx=ctx)
         b.init_case(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05, phases=(0.1, 0.2, 0.3), init="tgv", mach=0.3)
         return b
     ref_per = tgv()
@@ -136,7 +140,7 @@ def test_batch_with_bounded_blocks(ctx):
     per = tgv()
     per.configure(0.3, dtp, prm)
     blocks.append(per)
-    q = cfdx.Batch(cfdx.FP64, ctx)
+    q = cfdx.Batch(precision, ctx)
     for b in blocks:
         q.add(b)
     q.step(3, True)
