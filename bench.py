@@ -216,16 +216,34 @@ def tuned_tiles(cfdx, cdef, precision, mach, prm):
     return res
 
 
-def time_block(block, ctx, mach, dt, prm, steps, warmup, profile_steps):
-    """Warm-up, K timed steps bracketed by HIP events on the block's stream, then a separate
-    per-kernel-class timing pass."""
-    block.step(warmup, mach, dt, prm)
+def graph_steps(cfdx, block, ctx, tiles=None):
+    """A one-block batch around `block` whose step(n) replays a captured RK3-step graph: the
+    library's graph path for a single block (the mixed form's viscous/face overlap becomes graph
+    edges instead of stream events: 3-4 % faster mixed, FP64 unchanged). The block keeps its
+    state and run parameters; remove it (q.remove, q.close) to step it directly again."""
+    q = cfdx.Batch(block.precision, ctx)
+    q.add(block)
+    if tiles:
+        q.set_tiles(tuple(tiles))
+    return q
+
+
+def time_block(block, ctx, mach, dt, prm, steps, warmup, profile_steps, cfdx=None, tiles=None):
+    """Warm-up, K timed steps bracketed by HIP events on the block's stream (replayed as a
+    captured graph when cfdx is given, see graph_steps), then a separate per-kernel-class
+    timing pass of direct steps."""
+    q = graph_steps(cfdx, block, ctx, tiles) if cfdx is not None else None
+    step = (lambda n: q.step(n, True)) if q else (lambda n: block.step(n, mach, dt, prm))
+    step(warmup)
     ctx.sync()
     ctx.record(0)
-    block.step(steps, mach, dt, prm)
+    step(steps)
     ctx.record(1)
     ctx.sync()
     dev_ms = ctx.elapsed_ms(0, 1)
+    if q:
+        q.remove(block)
+        q.close()
     block.set_timing(True)
     block.step(profile_steps, mach, dt, prm)
     tk = block.timing()
@@ -398,12 +416,15 @@ def airfoil_c3(cfdx, steps: int):
         if dt is None:
             dt = b.cfl_dt(0.5)
         b.configure(0.5, dt, prm)
+        q = graph_steps(cfdx, b, ctx)
         ctx.sync()
         ctx.record(0)
-        b.step(steps, 0.5, dt, prm)
+        q.step(steps, True)
         ctx.record(1)
         ctx.sync()
         ms = ctx.elapsed_ms(0, 1)
+        q.remove(b)
+        q.close()
         res[name] = (b.get_flow(), n * steps / (ms / 1e3), ms / steps)
         b.close()
     errs = field_errors(res["mixed"][0], res["fp64"][0])
@@ -509,7 +530,8 @@ def main() -> int:
 
     ctx = block.ctx
     t0 = time.perf_counter()
-    dev_ms, tk = time_block(block, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps)
+    dev_ms, tk = time_block(block, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps, cfdx,
+                            tiles["tiles"] if tiles else None)
     wall_s = time.perf_counter() - t0
     ms_per_step = dev_ms / args.steps
     value = n * args.steps / (dev_ms / 1e3)
@@ -568,7 +590,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if precision == cfdx.FP64 else "f32 fluxes / f64 state",
         "data": "synthetic",
-        "config": dict(spec, parallelism="single GPU, one block", precision=args.precision),
+        "config": dict(spec, parallelism="single GPU, one block", precision=args.precision,
+                       stepping="captured RK3-step graph replays (one-block batch)"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_face (i, j, k sweeps; mean per launch)", "kernel_ms": face_ms,
@@ -591,7 +614,8 @@ def main() -> int:
         ob.configure(mach, dt, prm)
         if otiles:
             ob.set_tiles(tuple(otiles["tiles"]))
-        o_ms, o_tk = time_block(ob, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps)
+        o_ms, o_tk = time_block(ob, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps, cfdx,
+                                otiles["tiles"] if otiles else None)
         oflow = ob.get_flow()
         ob.close()
         mixed_flow, fp64_flow = (oflow, flow) if other == cfdx.MIXED else (flow, oflow)
