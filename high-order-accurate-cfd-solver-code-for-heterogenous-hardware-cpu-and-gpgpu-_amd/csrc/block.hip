@@ -241,17 +241,19 @@ uint32_t mon_triples(const BlockDev &B, const cfdx::blk::TileCfg &t)
     return cfdx::blk::tiles_of(B, 0, t) * (uint32_t)(tx * ty / 64) + (cfdx::blk::shell_count(B) + 63) / 64;
 }
 
-// the overlap of a stage's viscous kernel with its face sweeps, for mixed-precision work on a
-// context that allows it (the FP64 sweeps hold 3 waves per SIMD at ~160 VGPRs, which leaves no
-// registers for a viscous wave beside them: measured no gain, so FP64 stages run in order)
+// the overlap of a stage's viscous kernel with its face sweeps on a context that allows it: for
+// mixed-precision work always; for FP64 work only inside a captured graph (the FP64 sweeps hold
+// 3 waves per SIMD at ~160 VGPRs, which leaves few registers for a viscous wave beside them:
+// with stream events the overlap measured no gain, as graph edges 1 % at 128^3)
 cfdx::blk::Overlap overlap_of(const cfdx_ctx *ctx)
 {
     return cfdx::blk::Overlap{ctx->aux, ctx->fork, ctx->join, (uint32_t)std::max(ctx->n_cu, 1)};
 }
 
-const cfdx::blk::Overlap *overlap_if(const cfdx_ctx *ctx, int precision, const cfdx::blk::Overlap &ov)
+const cfdx::blk::Overlap *overlap_if(const cfdx_ctx *ctx, int precision, const cfdx::blk::Overlap &ov,
+                                     bool graph = false)
 {
-    return (ctx->overlap && precision == CFDX_MIXED) ? &ov : nullptr;
+    return (ctx->overlap && (precision == CFDX_MIXED || graph)) ? &ov : nullptr;
 }
 
 int run_stage(cfdx_block *b, int mode)
@@ -877,7 +879,7 @@ extern "C" int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph)
         // the second stream joins the capture through the fork event and is joined back before
         // each update, so the graph holds the overlap too
         for (int j = 0; j < 3; ++j)
-            cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, overlap_if(q->ctx, q->precision, ov));
+            cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, overlap_if(q->ctx, q->precision, ov, true));
         hipError_t e = hipStreamEndCapture(st, &q->graph);
         if (e != hipSuccess) { q->graph = nullptr; return cfdx::hip_fail(e, "hipStreamEndCapture"); }
         e = hipGraphInstantiate(&q->exec, q->graph, nullptr, nullptr, 0);

@@ -18,7 +18,7 @@ struct cfdx_ctx {
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     int n_cu = 0;
-    bool overlap = true;   // mixed-precision stages overlap the viscous kernel with the face sweeps
+    bool overlap = true;   // mixed stages (and FP64 ones in graphs) overlap the viscous kernel with the face sweeps
     std::atomic<int> refs{1};
 };
 

@@ -102,9 +102,10 @@ int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out);
 int cfdx_ctx_destroy(cfdx_ctx *ctx);
 int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream);
 void *cfdx_ctx_stream(cfdx_ctx *ctx);
-/* Overlap (no reference counterpart): mixed-precision block and batch stages run the viscous node
- * kernel on a second stream of the context, beside the face sweeps of the same stage (forked from
- * and joined back to the context's stream by events; a captured batch graph holds both). On by
+/* Overlap (no reference counterpart): mixed-precision block and batch stages, and the FP64 stages
+ * of a captured batch graph, run the viscous node kernel on a second stream of the context, beside
+ * the face sweeps of the same stage (forked from and joined back to the context's stream by events;
+ * a captured batch graph holds both). On by
  * default (environment CFDX_OVERLAP=0 turns the default off); timed passes (cfdx_block_set_timing)
  * always run in order. */
 int cfdx_ctx_set_overlap(cfdx_ctx *ctx, int on);
