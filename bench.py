@@ -201,6 +201,7 @@ def make_block(cfdx, cdef, precision, prm, ctx=None, host_case=None):
             b.set_case(host_case)
         else:
             b.init_case(**cdef["kw"])
+        b.configure(mach, 0.0, prm)
     dt = b.cfl_dt(0.5)
     b.configure(mach, dt, prm)
     return b, dt
@@ -341,8 +342,9 @@ def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int
             b = cfdx.Block(e, e, e, precision=precision, ctx=ctx)
             b.init_case(grid="sinusoid", length=(L,) * 3, amp=0.05 * L / e, phases=sp["phases"],
                         init="tgv" if sp["tgv"] else "vortex", mach=sp["mach"], center=(5.0, 5.0))
-            dt = b.cfl_dt(0.5)
             prm = cfdx.default_params(mach=sp["mach"], reyl=sp["reyl"])
+            b.configure(sp["mach"], 0.0, prm)
+            dt = b.cfl_dt(0.5)
             b.configure(sp["mach"], dt, prm)
             blocks.append((b, dt, prm, sp["mach"]))
         return blocks

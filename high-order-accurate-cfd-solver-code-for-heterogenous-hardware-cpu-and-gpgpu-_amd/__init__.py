@@ -129,6 +129,7 @@ EXPORTS = {
     "cfdx_block_get_flow": (C.c_int, [_vp, _vp]),
     "cfdx_block_init_case": (C.c_int, [_vp, C.POINTER(CaseDesc)]),
     "cfdx_block_cfl_dt": (C.c_int, [_vp, _d, C.POINTER(_d)]),
+    "cfdx_block_check_finite": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "cfdx_block_configure": (C.c_int, [_vp, _d, _d, C.POINTER(Params)]),
     "cfdx_block_step": (C.c_int, [_vp, _d, _d, C.c_int, C.POINTER(Params)]),
     "cfdx_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
@@ -373,15 +374,32 @@ def freestream(n: int, mach: float, aoa_deg: float = 0.0) -> np.ndarray:
     return f
 
 
-def cfl_dt(case: dict, cfl: float) -> float:
-    """dt = cfl / max_nodes sum_d (|u . grad xi_d| + c |grad xi_d|) on the initial state (computed once)."""
-    fl = case["FLOW"][1:]
-    m = case["metric"][1:, :9].reshape(-1, 3, 3)
+VISC_RADIUS = 1.372 * 1.372
+
+
+def cfl_dt(case: dict, cfl: float, params: "Params | None" = None) -> float:
+    """dt = cfl / max_nodes (r_c + r_v) on the initial state (computed once), the formula of
+    cfdx_block_cfl_dt: r_c = sum_d (|u . grad xi_d| + c |grad xi_d|); with viscous params,
+    r_v = 1.372^2 max(4/3, 1.4/Pr) mu / (rho Re) (sum_d |grad xi_d|)^2 (mu by Sutherland's law at
+    the node temperature 1.4 M^2 0.4 e with the params' Mach, T_inf). The reference itself runs a
+    fixed del_t (Main.cpp:47); this step is the build's choice for its workloads."""
+    # the owned nodes: a bounded case (tests/bounded.py) numbers them first, before its ghosts
+    n = case["layout"].n_own if "layout" in case else len(case["FLOW"]) - 1
+    fl = case["FLOW"][1:n + 1]
+    m = case["metric"][1:n + 1, :9].reshape(-1, 3, 3)
     vel = np.stack([fl["u"], fl["v"], fl["w"]], axis=1)
     c = np.sqrt(1.4 * fl["p"] / fl["rho"])
     s = np.zeros(len(fl))
+    sm = np.zeros(len(fl))
     for d in range(3):
-        s += np.abs(np.einsum("ni,ni->n", vel, m[:, d])) + c * np.linalg.norm(m[:, d], axis=1)
+        mm = np.linalg.norm(m[:, d], axis=1)
+        s += np.abs(np.einsum("ni,ni->n", vel, m[:, d])) + c * mm
+        sm += mm
+    if params is not None and params.viscous:
+        T = (1.4 * params.mach * params.mach * 0.4) * (fl["p"] / (0.4 * fl["rho"]))
+        suth = 110.4 / params.free_t
+        mu = (T * np.sqrt(T)) * ((1.0 + suth) / (T + suth))
+        s += VISC_RADIUS * max(4.0 / 3.0, 1.4 / params.pr) * (mu * (1.0 / params.reyl) / fl["rho"]) * sm * sm
     return float(cfl / s.max())
 
 
@@ -614,6 +632,12 @@ class Block:
         dt = _d(0.0)
         _check(_lib.cfdx_block_cfl_dt(self.h, cfl, C.byref(dt)))
         return dt.value
+
+    def check_finite(self) -> bool:
+        """True if every value of the current state (RK base) is finite (synchronous)."""
+        ok = C.c_int32(0)
+        _check(_lib.cfdx_block_check_finite(self.h, C.byref(ok)))
+        return bool(ok.value)
 
     def set_flow(self, flow: np.ndarray) -> None:
         flow = np.ascontiguousarray(flow)

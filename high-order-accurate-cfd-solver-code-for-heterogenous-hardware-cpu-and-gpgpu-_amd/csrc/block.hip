@@ -37,6 +37,11 @@ struct cfdx_block {
     uint32_t mon_cap = 0;         // partial triples allocated
     uint32_t mon_count = 0;       // partial triples written by the last step
     std::vector<cfdx_batch *> batches;   // batches holding the block (it leaves them when destroyed)
+    // recorded on a batch's stream when the block leaves it (cfdx_batch_remove): later work on the
+    // block waits for the batch's last launches over it, which may run on another context's stream
+    // (left_pending); cfdx_batch_add records it on the block's stream for the batch to wait on
+    hipEvent_t left = nullptr;
+    bool left_pending = false;
 };
 
 struct cfdx_batch {
@@ -57,6 +62,21 @@ struct cfdx_batch {
 };
 
 namespace {
+
+// Orders work on a block's own stream after the launches of a batch it has left.
+int wait_batches(cfdx_block *b)
+{
+    if (!b->left_pending) return CFDX_OK;
+    CFDX_HIP(hipStreamWaitEvent(b->ctx->stream, b->left, 0), "wait for the batch the block left");
+    return CFDX_OK;
+}
+
+// enter the block's context; its stream waits for a batch the block has left
+int enter_block(cfdx_block *b)
+{
+    CFDX_TRY(cfdx::enter(b->ctx));
+    return wait_batches(b);
+}
 
 size_t flux_size(int precision) { return precision == CFDX_MIXED ? sizeof(float) : sizeof(double); }
 
@@ -276,8 +296,11 @@ int run_stage(cfdx_block *b, int mode)
     return CFDX_OK;
 }
 
+// The captured step is destroyed only after every replay of it has finished on the batch's stream
+// (its kernel nodes hold copies of the launch arguments, which the runtime releases with it).
 void drop_graph(cfdx_batch *q)
 {
+    if (q->exec) (void)hipStreamSynchronize(q->ctx->stream);
     if (q->exec) (void)hipGraphExecDestroy(q->exec);
     if (q->graph) (void)hipGraphDestroy(q->graph);
     q->exec = nullptr;
@@ -444,7 +467,9 @@ extern "C" int cfdx_block_destroy(cfdx_block *b)
     if (!b) return CFDX_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    // a destroyed block leaves the batches that hold it, after their work on it has finished
+    // the launches of a batch the block has left, then those of the batches still holding it, end
+    // before its memory goes back to the pool
+    if (b->left_pending) (void)hipEventSynchronize(b->left);
     for (cfdx_batch *q : b->batches) {
         (void)hipStreamSynchronize(q->ctx->stream);
         auto it = std::find(q->blocks.begin(), q->blocks.end(), b);
@@ -454,6 +479,7 @@ extern "C" int cfdx_block_destroy(cfdx_block *b)
     b->batches.clear();
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (b->left) (void)hipEventDestroy(b->left);
     if (b->mem) cfdx::pool_give(b->ctx->device, b->mem, b->alloc);   // dB and dprefix live at its end
     if (b->mon) (void)hipFree(b->mon);
     cfdx_ctx *ctx = b->ctx;
@@ -466,7 +492,7 @@ extern "C" int cfdx_block_set_geometry(cfdx_block *b, const double *det, const c
     if (!b || !det || !metric || !deter) return cfdx::fail(CFDX_E_INVALID, "set_geometry: null argument");
     if (b->desc.periodic != 7)
         return cfdx::fail(CFDX_E_UNSUPPORTED, "set_geometry: a bounded block needs its ghost-node geometry (cfdx_block_set_geometry_soa)");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     const int ni = b->desc.ni, nj = b->desc.nj, nk = b->desc.nk;
     const long long n = (long long)ni * nj * nk;
     // face consistency: the minus face of a node must be the plus face of its -d neighbour
@@ -510,7 +536,7 @@ extern "C" int cfdx_block_set_geometry(cfdx_block *b, const double *det, const c
 extern "C" int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo)
 {
     if (!b || !geo) return cfdx::fail(CFDX_E_INVALID, "set_geometry_soa: null argument");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     CFDX_HIP(hipMemcpyAsync((void *)b->B.det, geo, (size_t)22 * b->B.n * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry_soa copy");
     cfdx::blk::geometry_f32(b->B, b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("k_geo32"));
@@ -529,7 +555,7 @@ extern "C" int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo)
 extern "C" int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow)
 {
     if (!b || !flow) return cfdx::fail(CFDX_E_INVALID, "set_flow: null argument");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     const long long n = (long long)b->desc.ni * b->desc.nj * b->desc.nk;
     std::vector<double> h((size_t)5 * n);
     for (long long g = 0; g < n; ++g) {
@@ -552,7 +578,7 @@ extern "C" int cfdx_block_set_flow(cfdx_block *b, const cfdx_flow *flow)
 extern "C" int cfdx_block_get_flow(cfdx_block *b, cfdx_flow *flow)
 {
     if (!b || !flow) return cfdx::fail(CFDX_E_INVALID, "get_flow: null argument");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     const long long n = (long long)b->desc.ni * b->desc.nj * b->desc.nk;
     // the viscous node-flux and face-flux arrays (contiguous, 27 flux-type values per stored node
     // >= 72 bytes + 16) are scratch between steps: the records are built there on the device,
@@ -574,7 +600,7 @@ extern "C" int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c)
     if (c->init < CFDX_INIT_FREESTREAM || c->init > CFDX_INIT_TGV)
         return cfdx::fail(CFDX_E_INVALID, "init_case: unknown initial state %d", c->init);
     if (b->desc.periodic != 7) return cfdx::fail(CFDX_E_UNSUPPORTED, "init_case: the sinusoidal map is for fully periodic blocks");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     cfdx::blk::init_geometry(b->B, c->length, c->amp, c->phase, b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_geometry"));
     cfdx::blk::geometry_f32(b->B, b->ctx->stream);
@@ -590,7 +616,7 @@ extern "C" int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c)
 extern "C" int cfdx_block_cfl_dt(cfdx_block *b, double cfl, double *dt)
 {
     if (!b || !dt || !(cfl > 0)) return cfdx::fail(CFDX_E_INVALID, "cfl_dt: null argument or cfl <= 0");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     // per-workgroup maxima in the face-flux scratch (at most 1025 doubles)
     double *part = (double *)b->B.FH[0][0];
     const int np = cfdx::blk::cfl_parts(b->B);
@@ -616,7 +642,7 @@ extern "C" int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nst
 {
     if (!b) return cfdx::fail(CFDX_E_INVALID, "block_step: null block");
     CFDX_TRY(check_params(params));
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     if (nsteps < 0) return cfdx::fail(CFDX_E_INVALID, "block_step: nsteps < 0");
     configure(b, Mach, del_t, params);
     CFDX_TRY(upload_single(b));
@@ -641,6 +667,21 @@ extern "C" int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nst
     return CFDX_OK;
 }
 
+extern "C" int cfdx_block_check_finite(cfdx_block *b, int32_t *finite)
+{
+    if (!b || !finite) return cfdx::fail(CFDX_E_INVALID, "check_finite: null argument");
+    CFDX_TRY(enter_block(b));
+    unsigned *cnt = (unsigned *)b->B.FH[0][0];   // counter in the face-flux scratch
+    CFDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), b->ctx->stream), "check_finite clear");
+    cfdx::blk::count_nonfinite(b->B.U0[0], (size_t)5 * b->B.n, cnt, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("k_nonfinite"));
+    unsigned bad = 0;
+    CFDX_HIP(hipMemcpyAsync(&bad, cnt, sizeof bad, hipMemcpyDeviceToHost, b->ctx->stream), "check_finite copy");
+    CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "check_finite sync");
+    *finite = bad ? 0 : 1;
+    return CFDX_OK;
+}
+
 extern "C" int cfdx_block_set_check(cfdx_block *b, int on)
 {
     if (!b) return cfdx::fail(CFDX_E_INVALID, "set_check: null block");
@@ -652,7 +693,7 @@ extern "C" int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *
 {
     if (!b || !L_host) return cfdx::fail(CFDX_E_INVALID, "block_rhs: null argument");
     CFDX_TRY(check_params(params));
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     const long long n = b->B.n;
     double *Ld = nullptr;
     CFDX_HIP(hipMalloc((void **)&Ld, (size_t)5 * n * 8), "block_rhs hipMalloc");
@@ -695,7 +736,7 @@ extern "C" int cfdx_block_timing(cfdx_block *b, double *ms_visc, double *ms_face
 extern "C" int cfdx_block_set_monitor(cfdx_block *b, int on)
 {
     if (!b) return cfdx::fail(CFDX_E_INVALID, "set_monitor: null block");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     if (on && !b->mon) {
         uint32_t cap = 0;
         for (int v = 0; v < cfdx::blk::N_NODE; ++v) {
@@ -718,7 +759,7 @@ extern "C" int cfdx_block_residual(cfdx_block *b, double out[3])
     if (!b || !out) return cfdx::fail(CFDX_E_INVALID, "residual: null argument");
     if (!b->B.mon) return cfdx::fail(CFDX_E_INVALID, "residual: the monitor is off (cfdx_block_set_monitor)");
     if (b->mon_count == 0) return cfdx::fail(CFDX_E_INVALID, "residual: no step with the monitor on yet");
-    CFDX_TRY(cfdx::enter(b->ctx));
+    CFDX_TRY(enter_block(b));
     double *dout = b->mon + 3 * (size_t)b->mon_cap;
     cfdx::blk::monitor_reduce(b->mon, b->mon_count, dout, b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("monitor_reduce"));
@@ -831,6 +872,14 @@ extern "C" int cfdx_batch_add(cfdx_batch *q, cfdx_block *b)
     if (b->ctx->device != q->ctx->device) return cfdx::fail(CFDX_E_INVALID, "batch_add: block on another device");
     if (!b->configured) return cfdx::fail(CFDX_E_INVALID, "batch_add: configure the block (cfdx_block_configure) first");
     if (std::find(q->blocks.begin(), q->blocks.end(), b) != q->blocks.end()) return cfdx::fail(CFDX_E_INVALID, "batch_add: block already in the batch");
+    CFDX_TRY(enter_block(b));
+    // the batch's launches start after the work queued on the block's own stream (its set-up)
+    if (b->ctx->stream != q->ctx->stream) {
+        if (!b->left) CFDX_HIP(hipEventCreateWithFlags(&b->left, hipEventDisableTiming), "batch_add event");
+        CFDX_HIP(hipEventRecord(b->left, b->ctx->stream), "batch_add record");
+        CFDX_HIP(hipStreamWaitEvent(q->ctx->stream, b->left, 0), "batch_add wait");
+        b->left_pending = false;
+    }
     q->blocks.push_back(b);
     b->batches.push_back(q);
     q->dirty = true;
@@ -842,6 +891,11 @@ extern "C" int cfdx_batch_remove(cfdx_batch *q, cfdx_block *b)
     if (!q || !b) return cfdx::fail(CFDX_E_INVALID, "batch_remove: null argument");
     auto it = std::find(q->blocks.begin(), q->blocks.end(), b);
     if (it == q->blocks.end()) return cfdx::fail(CFDX_E_INVALID, "batch_remove: block not in the batch");
+    CFDX_TRY(cfdx::enter(q->ctx));
+    if (!b->left) CFDX_HIP(hipEventCreateWithFlags(&b->left, hipEventDisableTiming), "batch_remove event");
+    // the batch's launches so far may still read and write the block: its later work waits for them
+    CFDX_HIP(hipEventRecord(b->left, q->ctx->stream), "batch_remove record");
+    b->left_pending = true;
     q->blocks.erase(it);
     auto ib = std::find(b->batches.begin(), b->batches.end(), q);
     if (ib != b->batches.end()) b->batches.erase(ib);

@@ -126,10 +126,17 @@ __global__ void k_init(BlockDev B, InitArgs a)
     B.Us[0][g] = rho; B.Us[1][g] = u; B.Us[2][g] = v; B.Us[3][g] = w; B.Us[4][g] = p;
 }
 
-// CFL spectral radius max over nodes of sum_d |u . grad xi_d| + c |grad xi_d|, from the RK base
-// U0 = det Q (valid between steps); per-workgroup maxima into part[], then one workgroup reduces
-// them into part[nparts]
+// Stable-step radius max over nodes of r_c + r_v, from the RK base U0 = det Q (valid between
+// steps); per-workgroup maxima into part[], then one workgroup reduces them into part[nparts].
+//   r_c = sum_d |u . grad xi_d| + c |grad xi_d|                         (convection)
+//   r_v = VISC_RADIUS * mu / (rho Re) * (sum_d |grad xi_d|)^2            (viscous blocks only)
+// r_v bounds the diffusion operator of the viscous terms: the 4th-order central first derivative
+// (unit computational spacing) has a symbol of modulus at most 1.372, applied twice along any pair
+// of directions through the metric rows, times the largest diffusivity factor max(4/3, 1.4 / Pr)
+// of the stresses and the heat flux; mu is Sutherland's law at the node temperature (the viscous
+// kernel's), Re, Pr, T_inf and the kernel Mach number those of the block's parameters.
 constexpr int RED = 256;
+constexpr double VISC_RADIUS = 1.372 * 1.372;
 
 __device__ __forceinline__ double block_max(double v, double *sh)
 {
@@ -155,10 +162,19 @@ __global__ void __launch_bounds__(RED) k_cfl_part(BlockDev B, double *part)
         const double u = (idet * B.U0[1][g]) * ir, v = (idet * B.U0[2][g]) * ir, w = (idet * B.U0[3][g]) * ir;
         const double p = 0.4 * ((idet * B.U0[4][g]) - 0.5 * rho * (u * u + v * v + w * w));
         const double c = sqrt(1.4 * p / rho);
-        double s = 0.0;
+        double s = 0.0, sm = 0.0;
         for (int d = 0; d < 3; ++d) {
             const double mx = B.m[3 * d][g], my = B.m[3 * d + 1][g], mz = B.m[3 * d + 2][g];
-            s += fabs(u * mx + v * my + w * mz) + c * sqrt(mx * mx + my * my + mz * mz);
+            const double mm = sqrt(mx * mx + my * my + mz * mz);
+            s += fabs(u * mx + v * my + w * mz) + c * mm;
+            sm += mm;
+        }
+        if (B.viscous) {
+            const cfdx::dev::ViscParams &vp = B.vp;
+            const double T = vp.tfac * (p / (0.4 * rho));          // 1.4 M^2 0.4 e
+            const double mu = (T * sqrt(T)) * (vp.suth1 / (T + vp.suth));
+            const double diff = fmax(4.0 / 3.0, 1.4 / vp.pr);
+            s += VISC_RADIUS * diff * (mu * vp.inv_reyl / rho) * (sm * sm);
         }
         r = fmax(r, s);
     }

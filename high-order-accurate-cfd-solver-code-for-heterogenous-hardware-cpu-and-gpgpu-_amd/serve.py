@@ -80,6 +80,7 @@ class Result:
     finished: float
     dt: float
     flow: np.ndarray | None
+    finite: bool = True              # the final state holds no NaN / infinity (checked on the device)
 
     @property
     def latency(self) -> float:
@@ -114,6 +115,7 @@ def make_block(req: Request, precision: int, ctx: Ctx, prep: dict | None = None)
             amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
             b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
                         beta=req.beta, center=req.center)
+            b.configure(req.mach, 0.0, req.params())      # the stable step uses the request's Re
         dt = b.cfl_dt(req.cfl)
         b.configure(req.mach, dt, req.params())
     except BaseException:
@@ -126,9 +128,10 @@ def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) ->
     """The request on its own (single-block steps), for equivalence checks."""
     b, dt = make_block(req, precision, ctx)
     b.step(req.nsteps, req.mach, dt, req.params())
+    finite = b.check_finite()
     flow = b.get_flow() if keep_flow else None
     b.close()
-    return Result(req.rid, 0.0, 0.0, 0.0, dt, flow)
+    return Result(req.rid, 0.0, 0.0, 0.0, dt, flow, finite)
 
 
 class Server:
@@ -266,9 +269,13 @@ class Server:
                 # results of the requests that finished, copied on the set-up stream during the chunk
                 for a in done:
                     b = a["block"]
+                    # a diverged solve is reported as such (Result.finite, report()["nonfinite"]),
+                    # not returned as a NaN field
+                    finite = b.check_finite()
                     flow = b.get_flow() if self.keep_flow else None
                     used -= a["bytes"]
-                    results[a["req"].rid] = Result(a["req"].rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow)
+                    results[a["req"].rid] = Result(a["req"].rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow,
+                                                   finite)
                     retired.append(b)
                 if not (active or admitted or inflight) and pending and not queue:
                     for b in retired:
@@ -315,7 +322,8 @@ def report(requests: list[Request], results: dict[int, Result]) -> dict:
     lat = np.array([results[r.rid].latency for r in requests])
     cells = float(sum(r.nodes * r.nsteps for r in requests))
     span = max(res.finished for res in results.values()) - min(r.arrival for r in requests)
-    return {"requests": len(requests), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+    return {"requests": len(requests), "nonfinite": sum(1 for r in requests if not results[r.rid].finite),
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "max_s": float(lat.max()), "makespan_s": span,
             "cell_updates": cells, "cell_updates_per_s": cells / span if span > 0 else float("nan")}
 

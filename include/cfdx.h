@@ -229,9 +229,16 @@ typedef struct {
 } cfdx_case_desc;
 int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c);
 
-/* dt = cfl / max over nodes of sum_d (|u . grad xi_d| + c |grad xi_d|) of the current state.
+/* Stable time step of the current state (no counterpart in the reference, which runs a fixed
+ * del_t, Main.cpp:47): dt = cfl / max over nodes of (r_c + r_v) with the convective radius
+ * r_c = sum_d (|u . grad xi_d| + c |grad xi_d|) and, for a block configured viscous, the viscous
+ * radius r_v = 1.372^2 max(4/3, 1.4/Pr) mu / (rho Re) (sum_d |grad xi_d|)^2 (mu by Sutherland's law
+ * at the node temperature, with the Mach, Re, Pr and T_inf of the block's last configure).
  * Synchronous. */
 int cfdx_block_cfl_dt(cfdx_block *b, double cfl, double *dt);
+/* Non-finite check of the block's current state (no counterpart in the reference): *finite = 1 if
+ * every value of the RK base U0 (all stored nodes) is finite, else 0. One device pass; synchronous. */
+int cfdx_block_check_finite(cfdx_block *b, int32_t *finite);
 
 /* Run parameters of the block (Weno-argument Mach for t = gamma M^2 p/rho, fixed del_t, physical
  * constants, viscous switch), used by cfdx_block_step and by batched steps. */

@@ -57,7 +57,9 @@ def test_device_init_matches_host(ctx, kw):
     for c in range(5):
         err = np.abs(Ld[c] - Lh[c]).max() / scale
         assert err <= 1e-11, (c, err)
-    dt_host = cfdx.cfl_dt(case, 0.5)
+    # the same stable step (convective + viscous radius) on both sides, with the block's parameters
+    dev.configure(kw["mach"], 0.0, prm)
+    dt_host = cfdx.cfl_dt(case, 0.5, prm)
     dt_dev = dev.cfl_dt(0.5)
     assert abs(dt_dev - dt_host) <= 1e-13 * dt_host, (dt_dev, dt_host)
 
