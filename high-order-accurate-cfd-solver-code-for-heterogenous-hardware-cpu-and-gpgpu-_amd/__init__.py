@@ -575,6 +575,7 @@ class Block:
         _check(_lib.cfdx_block_create(self.ctx.h, C.byref(self.desc), C.byref(h)))
         self.h = h
         self.n = ni * nj * nk
+        self.dt = 0.0       # del_t of the last configure / step
 
     @classmethod
     def of(cls, desc: BlockDesc, ctx: Ctx | None = None) -> "Block":
@@ -652,9 +653,11 @@ class Block:
 
     def configure(self, mach: float, dt: float, params: Params) -> None:
         _check(_lib.cfdx_block_configure(self.h, mach, dt, C.byref(params)))
+        self.dt = dt
 
     def step(self, nsteps: int, mach: float, dt: float, params: Params) -> None:
         _check(_lib.cfdx_block_step(self.h, mach, dt, nsteps, C.byref(params)))
+        self.dt = dt
 
     def rhs(self, params: Params) -> np.ndarray:
         out = np.zeros((5, self.n))

@@ -59,6 +59,7 @@ struct cfdx_batch {
     cfdx::blk::TileCfg tiles;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t graph_gen = 0;       // context generation the graph was captured under
 };
 
 namespace {
@@ -928,7 +929,9 @@ extern "C" int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph)
                 cfdx::blk::stage(q->L, q->precision, j, st, q->any_visc, nullptr, overlap_if(q->ctx, q->precision, ov));
         return cfdx::check_launch("batch stage");
     }
+    if (q->exec && q->graph_gen != q->ctx->gen.load()) drop_graph(q);   // overlap switch or stream changed
     if (!q->exec) {
+        q->graph_gen = q->ctx->gen.load();
         CFDX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         // the second stream joins the capture through the fork event and is joined back before
         // each update, so the graph holds the overlap too

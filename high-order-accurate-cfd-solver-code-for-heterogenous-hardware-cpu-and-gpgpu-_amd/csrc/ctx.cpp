@@ -181,13 +181,17 @@ extern "C" int cfdx_ctx_set_stream(cfdx_ctx *ctx, void *stream)
         CFDX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
         ctx->owns_stream = true;
     }
+    ctx->gen.fetch_add(1);
     return CFDX_OK;
 }
 
 extern "C" int cfdx_ctx_set_overlap(cfdx_ctx *ctx, int on)
 {
     CFDX_TRY(cfdx::enter(ctx));
-    ctx->overlap = on != 0;
+    if (ctx->overlap != (on != 0)) {
+        ctx->overlap = on != 0;
+        ctx->gen.fetch_add(1);   // captured batch steps hold the old fork / join structure
+    }
     return CFDX_OK;
 }
 

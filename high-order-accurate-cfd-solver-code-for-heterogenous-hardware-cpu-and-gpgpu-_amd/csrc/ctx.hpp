@@ -19,6 +19,9 @@ struct cfdx_ctx {
     hipEvent_t fork = nullptr, join = nullptr;
     int n_cu = 0;
     bool overlap = true;   // mixed stages (and FP64 ones in graphs) overlap the viscous kernel with the face sweeps
+    // bumped by every change that a captured batch graph depends on (overlap switch, stream): a batch
+    // re-captures its step when the generation it captured under is not the current one
+    std::atomic<uint64_t> gen{1};
     std::atomic<int> refs{1};
 };
 

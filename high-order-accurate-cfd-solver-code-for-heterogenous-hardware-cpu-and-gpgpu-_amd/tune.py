@@ -1,6 +1,6 @@
 """Tile autotuner for the block kernels.
 
-Every kernel class (face sweep along i, along j, along k; the node kernels) has a few compiled
+Every kernel class (sweep along i, along j, along k; the viscous node kernel) has a few compiled
 tile shapes (`cfdx.tile_variants`). The tuner times them on the actual grid with HIP events on
 the block's stream and keeps the fastest: a coordinate search, one kernel class at a time with
 the others fixed at the best found so far (the classes are separate launches, so their times
@@ -51,10 +51,17 @@ class TileCache:
                 pass
 
 
+def tables_tag() -> str:
+    """The compiled tile shapes, so that a cached choice is dropped when the library's tables change."""
+    shapes = [tile_variants(kind) for kind in range(3)]
+    return "t" + "-".join("_".join(f"{a}x{b}" for a, b in kind) for kind in shapes)
+
+
 def block_key(ni: int, nj: int, nk: int, precision: int, viscous: bool, periodic: int = 7) -> str:
-    """Cache key of a block: shape, precision, viscous switch and (bounded blocks) the periodic mask."""
+    """Cache key of a block: shape, precision, viscous switch, (bounded blocks) the periodic mask and
+    the library's tile tables."""
     tail = "" if periodic == 7 else f":per{periodic}"
-    return f"block:{ni}x{nj}x{nk}:p{precision}:v{int(bool(viscous))}{tail}"
+    return f"block:{ni}x{nj}x{nk}:p{precision}:v{int(bool(viscous))}{tail}:{tables_tag()}"
 
 
 def _counts() -> list[int]:
@@ -98,6 +105,8 @@ def tune_batch(batch: Batch, ctx: Ctx, key: str | None = None, steps: int = 2, c
                graph: bool = True, force: bool = False) -> dict:
     """Chooses and sets the tile variants and the persistent grid size of a batch by the time of
     whole batched steps; returns {"tiles", "grid_cap", "times_ms", "cached"}."""
+    if key:
+        key = f"{key}:{tables_tag()}"
     if cache is not None and key and not force:
         hit = cache.get(key)
         if hit:

@@ -107,7 +107,8 @@ void *cfdx_ctx_stream(cfdx_ctx *ctx);
  * the face sweeps of the same stage (forked from and joined back to the context's stream by events;
  * a captured batch graph holds both). On by
  * default (environment CFDX_OVERLAP=0 turns the default off); timed passes (cfdx_block_set_timing)
- * always run in order. */
+ * always run in order. Changing it (or the stream, cfdx_ctx_set_stream) makes the batches of the
+ * context capture their step again at their next cfdx_batch_step. */
 int cfdx_ctx_set_overlap(cfdx_ctx *ctx, int on);
 int cfdx_sync(cfdx_ctx *ctx);
 

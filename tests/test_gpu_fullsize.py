@@ -11,11 +11,10 @@
   and mixed;
 - C3 (384x128x32 NACA 0012 C-grid): 20 steps from the free stream in mixed precision against
   FP64 (per-field relative L2 <= 1e-5) with a finite state;
-- C2's full 200 steps against the oracle (about two minutes of CPU oracle work on 16 cores): run
-  only with CFDX_LONG_TESTS=1.
+- C2's full 200 steps against the oracle (about two minutes of CPU oracle work on 16 cores),
+  1e-12 relative.
 """
 import math
-import os
 
 import numpy as np
 import pytest
@@ -101,7 +100,6 @@ def test_c2_full_size_step_rhs_and_conservation(ctx, orc):
         assert (total <= 1e-11 * scale).all(), (precision, total, scale)
 
 
-@pytest.mark.skipif(os.environ.get("CFDX_LONG_TESTS") != "1", reason="C2's 200 oracle steps take minutes; CFDX_LONG_TESTS=1")
 def test_c2_200_steps(ctx, orc):
     import pyoracle
     print("C2 128^3 TGV, 200 steps vs oracle", flush=True)

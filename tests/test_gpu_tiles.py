@@ -138,12 +138,11 @@ def test_overlap_gives_the_same_bits(ctx, graph):
         b1.configure(0.2, b1.cfl_dt(0.5), prm)
         c = cfdx.Block(48, 16, 4, precision=cfdx.MIXED, ctx=ctx, desc=d)
         c.set_geometry_soa(geo)
-        # Re 1e5 (the C-grid's wall spacing is 1e-4 chord) and half the CFL step: the impulsive start
-        # on this small stretched grid
-        prm_c = cfdx.default_params(mach=0.3, reyl=1e5)
-        c.configure(0.3, 0.0, prm_c)
+        # Re 1200 on this C-grid (wall spacing 1e-4 chord) at the full stable step: the viscous bound of
+        # cfdx_block_cfl_dt keeps the impulsive start finite
+        c.configure(0.3, 0.0, prm)
         c.set_flow(cfdx.freestream(48 * 16 * 4, 0.3, 3.0))
-        c.configure(0.3, 0.5 * c.cfl_dt(0.5), prm_c)
+        c.configure(0.3, c.cfl_dt(0.5), prm)
         q.add(b1)
         q.add(c)
         q.step(4, graph)
@@ -157,6 +156,81 @@ def test_overlap_gives_the_same_bits(ctx, graph):
         for f in ("rho", "u", "v", "w", "p"):
             assert np.array_equal(a[f], r[f]), f
             assert np.isfinite(a[f]).all()
+
+
+def test_overlap_switch_recaptures(ctx):
+    """Switching the overlap after a batch captured its step: the next step captures again (the graph
+    holds the fork / join of the old setting); the state stays bitwise that of the block alone."""
+    prm = cfdx.default_params(mach=0.2, reyl=900.0)
+    alone = ragged(ctx, cfdx.MIXED, 24, 16, 8)
+    dt = alone.cfl_dt(0.5)
+    alone.configure(0.2, dt, prm)
+    alone.step(6, 0.2, dt, prm)
+    want = alone.get_flow()
+    b = ragged(ctx, cfdx.MIXED, 24, 16, 8)
+    b.configure(0.2, dt, prm)
+    q = cfdx.Batch(cfdx.MIXED, ctx)
+    q.add(b)
+    try:
+        ctx.set_overlap(True)
+        q.step(2, True)
+        ctx.set_overlap(False)
+        q.step(2, True)
+        ctx.set_overlap(True)
+        q.step(2, True)
+        ctx.sync()
+    finally:
+        ctx.set_overlap(True)
+    got = b.get_flow()
+    for f in FIELDS:
+        assert np.array_equal(got[f], want[f]), f
+    q.close()
+    b.close()
+    alone.close()
+
+
+def test_remove_then_destroy_across_contexts(ctx):
+    """Blocks on a second context in a batch of the first: a block removed from the batch and
+    destroyed at once (its memory back in the pool while the batch's launches may still run), then
+    a new block created and added, gives every block bitwise its result alone."""
+    io = cfdx.Ctx(0)
+    prm = cfdx.default_params(mach=0.2, reyl=900.0)
+
+    def make(shape):
+        b = ragged(io, cfdx.MIXED, *shape)
+        b.configure(0.2, 0.0, prm)
+        b.configure(0.2, b.cfl_dt(0.5), prm)
+        return b
+
+    shapes = [(24, 16, 8), (19, 13, 11), (20, 16, 8)]
+    alone = []
+    for sh, n in zip(shapes, (6, 3, 3)):
+        b = make(sh)
+        b.step(n, 0.2, b.dt, prm)
+        alone.append(b.get_flow())
+        b.close()
+    q = cfdx.Batch(cfdx.MIXED, ctx)
+    a, b = make(shapes[0]), make(shapes[1])
+    q.add(a)
+    q.add(b)
+    q.step(3, True)               # asynchronous on the batch's stream
+    q.remove(b)
+    got_b = b.get_flow()          # waits for the batch's launches over b
+    b.close()
+    c = make(shapes[2])           # may take b's memory from the pool
+    q.add(c)
+    q.step(3, True)
+    ctx.sync()
+    for x, want in ((a, alone[0]), (c, alone[2])):
+        got = x.get_flow()
+        for f in FIELDS:
+            assert np.array_equal(got[f], want[f]), f
+    for f in FIELDS:
+        assert np.array_equal(got_b[f], alone[1][f]), f
+    q.close()
+    a.close()
+    c.close()
+    io.close()
 
 
 def test_destroyed_block_leaves_its_batches(ctx):
@@ -177,8 +251,9 @@ def test_destroyed_block_leaves_its_batches(ctx):
     got = a.get_flow()
     q.close()
     alone = ragged(ctx, cfdx.MIXED, 24, 16, 8)
-    alone.configure(0.2, alone.cfl_dt(0.5), prm)
-    alone.step(4, 0.2, alone.cfl_dt(0.5), prm)
+    dt = alone.cfl_dt(0.5)        # the step the batch's block took (computed before its configure)
+    alone.configure(0.2, dt, prm)
+    alone.step(4, 0.2, dt, prm)
     want = alone.get_flow()
     for f in ("rho", "u", "v", "w", "p"):
         assert np.array_equal(got[f], want[f]), f
