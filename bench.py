@@ -1,23 +1,29 @@
 """Throughput of the per-step RHS + SSP-RK3 update on one MI355X.
 
-Workload (default): 128^3 periodic curvilinear Taylor-Green vortex, FP64 Navier-Stokes
-with the reference's mixed-time-scale LES model, Mach 0.1, Re 1600, CFL 0.5 (dt fixed
-from the initial state), full RHS (characteristic WCNS + LLF + 6th-order correction +
-viscous/SGS fluxes) and RK3. A step = one full SSP-RK3 step over the whole grid with
-the state resident in HBM. Prints one JSON line.
+Headline workload: 256^3 periodic curvilinear Taylor-Green vortex (the 128^3 case's seeded
+sinusoidal map, amplitude 0.05 of the spacing), FP64 Navier-Stokes with the reference's
+mixed-time-scale LES model, Mach 0.1, Re 1600, CFL 0.5 (dt fixed from the initial state), full
+RHS (characteristic WCNS + LLF + 6th-order correction + viscous/SGS fluxes) and RK3: one GPU's
+share of a 512^3 run. A step = one full SSP-RK3 step over the whole grid with the state resident
+in HBM; value = interior nodes x steps / device time. Prints one JSON line.
 
-Beside the headline the same JSON line carries (unless --legs drops them): the other precision
-on the same case with per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the same flow at 256^3
-in FP64 (one GPU's share of a 512^3 run, "one_gpu_share_256"), the 64-cube batched
-mixed-precision graph step ("batched_c4"), the NACA 0012 C-grid in mixed and plain FP64 with
-per-field errors ("airfoil_c3"), and a bounded sample of the Poisson stream of NACA C-grid solves
-served by continuous batching vs plain FP64 one at a time ("serving": p50 / p99 latency,
-throughput).
+`roofline.achieved` = 848 algorithmic HBM bytes per cell-update x cell-updates/s; the counter
+figures of the step (measured HBM bytes and VALU instructions per kernel, from the committed
+rocprofv3 summary of the workload) name the bound; the face sweep's own figures (HIP-event timed
+in this run) are a sub-field.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|ns256|c1|c3] [--precision fp64|mixed]
-                    [--legs compare,ns256,c4,c3,serve|none] [--no-cpu] [--no-tune]
+Legs on the same line (unless --legs drops them): the other precision on the headline grid with
+per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the 128^3 case in FP64 and mixed ("c2"),
+the 128^3 case through the drop-in kernels ("dropin_path"), the 64-cube batched mixed graph step
+("batched_c4"), the NACA 0012 C-grid in mixed and FP64 with per-field errors ("airfoil_c3"), and
+the 200-request Poisson stream of NACA C-grid solves served by continuous batching vs plain FP64
+one at a time ("serving": p50 / p99 latency, throughput). cpu_baseline: the reference's kernels
+on the host cores for the 64^3 vortex (100 steps), the 128^3 case (20) and 256^3 (2), all cores
+and then one thread, each beside the GPU block path on the same case, dt and step count (rates
+side by side and the per-field differences of the two final states).
 
---workload c3 makes the NACA 0012 C-grid block (384x128x32, free stream start) the timed block.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ns256|c2|c1|c3] [--precision fp64|mixed]
+                    [--legs compare,c2,dropin,c4,c3,serve|none] [--cpu c1,c2,ns256|none] [--no-tune]
 """
 from __future__ import annotations
 
@@ -34,6 +40,7 @@ PKG_DIR = os.path.join(ROOT, "high-order-accurate-cfd-solver-code-for-heterogeno
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 # VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD
 VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 ALG_BYTES_PER_CELL_UPDATE = 848  # minimal FP64 SoA traffic of one RK3 step (3 stages)
@@ -74,17 +81,13 @@ def _import_pkg():
 
 
 def workload(name: str):
-    import numpy as np
-    rng = np.random.default_rng(20261202)
+    """Timed-block definition of a workload: (config spec, case definition, params keywords, viscous)."""
+    from cfdx import workloads
     if name in ("c2", "ns256"):
         n = 128 if name == "c2" else 256
-        h = 2 * math.pi / n
-        phases = tuple(rng.uniform(0, 2 * math.pi, 3))
         spec = dict(workload=f"taylor_green_les_{n}^3_curvilinear", grid=[n, n, n], mach=0.1, re=1600.0,
                     sgs="mixed_time_scale(C=0.03,C_T=10)", cfl=0.5, map="sinusoidal amp=0.05h seeded phases")
-        case = dict(ni=n, nj=n, nk=n, kw=dict(grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * h,
-                                               phases=phases, init="tgv", mach=0.1))
-        return spec, case, dict(mach=0.1, reyl=1600.0), True
+        return spec, dict(ni=n, nj=n, nk=n, kw=workloads.tgv_case_kw(n)), dict(mach=0.1, reyl=1600.0), True
     if name == "c3":
         spec = dict(workload="naca0012_cgrid_384x128x32_les", grid=[384, 128, 32], mach=0.5, aoa_deg=2.0, re=1e5,
                     sgs="mixed_time_scale(C=0.03,C_T=10)", cfl=0.5,
@@ -94,20 +97,19 @@ def workload(name: str):
         return spec, case, dict(mach=0.5, reyl=1e5), True
     if name == "c1":
         spec = dict(workload="isentropic_vortex_euler_64^3", grid=[64, 64, 64], mach=0.5, beta=5.0, cfl=0.5)
-        case = dict(ni=64, nj=64, nk=64, kw=dict(grid="box", length=(10.0,) * 3, init="vortex", mach=0.5))
-        return spec, case, dict(), False
+        return spec, dict(ni=64, nj=64, nk=64, kw=workloads.vortex_case_kw(64)), dict(), False
     raise SystemExit(f"unknown workload {name}")
 
 
-def profile_counts(cfdx, workload_name: str, precision: str, tiles=None):
-    """Per-launch HBM bytes and VALU counts of the face kernels, from the committed rocprofv3
-    summary of this workload (profiles/block_kernels_<workload>_<precision>.md). The summary
-    holds one machine-readable row per kernel and tile variant,
-    `| kernel<S,DIR,TA,TB> | launches | avg_us | hbm_bytes | valu_instr | fp64_flop | fp32_flop |`;
-    the rows of the variants this run uses (tiles = face i, j, k, node variant indices) are read."""
+def profile_rows(workload_name: str, precision: str):
+    """Per-kernel rows of the committed rocprofv3 summary of this workload
+    (profiles/block_kernels_<workload>_<precision>.md): one machine-readable row per kernel and
+    tile variant, `| kernel<...> | launches | avg_us | hbm_bytes | valu_instr | fp64_flop | fp32_flop |`,
+    hbm_bytes = FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration, valu_instr = SQ_INSTS_VALU
+    (wave instructions), per launch; "-" where the counter runs did not launch that variant."""
     path = os.path.join(ROOT, "profiles", f"block_kernels_{workload_name}_{precision}.md")
     if not os.path.exists(path):
-        return None
+        return None, path
     rows = {}
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
@@ -118,66 +120,139 @@ def profile_counts(cfdx, workload_name: str, precision: str, tiles=None):
                                       fp64_flop=float(cells[5]), fp32_flop=float(cells[6]))
             except ValueError:
                 continue
+    return rows, path
+
+
+def step_kernels(cfdx, precision: str, tiles, viscous: bool = True):
+    """Kernel names (with template arguments) of one RK stage of a periodic block with tile
+    variants `tiles` = (face i, face j, face k, node variant indices)."""
     t = tuple(tiles) if tiles else (0, 0, 0, 0)
     S = "double" if precision == "fp64" else "float"
     names = []
+    nx, ny = cfdx.tile_variants(2)[t[3]]
+    if viscous:
+        names.append(f"k_visc<{S},{nx},{ny}>")
     for d in range(3):
         ta, tb = cfdx.tile_variants(0 if d == 0 else 1)[t[d]]
         names.append(f"k_face<{S},{d},{ta},{tb}>")
-    face, used = [], []
-    for d, k in enumerate(names):
-        if k not in rows:   # the profiled run tuned another variant of this class: take its counted row
-            k = next((r for r in rows if r.startswith(f"k_face<{S},{d},")), None)
-            if k is None:
+    names.append(f"k_update<{S},{nx},{ny}>")
+    return names
+
+
+def counter_figures(cfdx, workload_name: str, precision: str, tiles, viscous: bool, nodes: int):
+    """Counter figures of one RK stage from the committed profile: per kernel the measured HBM
+    bytes and VALU wave instructions per launch (the profiled run's own tuned variants when this
+    run tuned others of the same class), and the stage totals."""
+    rows, path = profile_rows(workload_name, precision)
+    if not rows:
+        return None
+    want = step_kernels(cfdx, precision, tiles, viscous)
+    per, same = {}, True
+    for k in want:
+        r = rows.get(k)
+        if r is None or r["hbm_bytes"] != r["hbm_bytes"]:
+            # the counter runs used another variant of this class: take the counted one
+            cls = k[:k.index("<")] + "<" + ",".join(k[k.index("<") + 1:].split(",")[:2 if k.startswith("k_face") else 1])
+            k2 = next((n for n, v in rows.items() if n.startswith(cls) and v["hbm_bytes"] == v["hbm_bytes"]), None)
+            if k2 is None:
                 return None
-        face.append(rows[k])
-        used.append(k)
-    mean = {k: sum(f[k] for f in face) / 3 for k in ("avg_us", "hbm_bytes", "valu_instr", "fp64_flop", "fp32_flop")}
-    mean["source"] = os.path.relpath(path, ROOT)
-    mean["rows"] = used
-    mean["same_variants"] = used == names
-    return mean
+            same = False
+            r = rows[k2]
+            k = k2
+        per[k] = r
+    stage_bytes = sum(r["hbm_bytes"] for r in per.values())
+    stage_valu = sum(r["valu_instr"] for r in per.values())
+    stage_us = sum(r["avg_us"] for r in per.values())
+    return {"source": os.path.relpath(path, ROOT), "kernels": per, "same_variants": same,
+            "stage_hbm_bytes": stage_bytes, "stage_valu_instr": stage_valu, "stage_profiled_us": stage_us,
+            "hbm_bytes_per_cell_update": 3 * stage_bytes / nodes}
 
 
-def cpu_baseline(cfdx, max_seconds: float = 15.0):
-    """The reference's own kernels (oracle/_ref, gcc -O3, OpenMP) or the restatement, on a
-    bounded sample of the same workload: a 64^3 block of the curvilinear TGV, whole RK3 steps."""
+def cpu_baseline(cfdx, which=("c1", "c2", "ns256"), single_thread=("c1", "c2", "ns256"), gpu=True):
+    """The reference's CPU path, timed on this host beside the GPU on the same inputs: the
+    reference's own kernels compiled as C (oracle/_ref, gcc -O3 -march=native -fopenmp, OpenMP over
+    nodes; the restatement when that library is absent) stepping the reference's AoS arrays through
+    its stage sequence. Configurations: C1 (64^3 vortex, Euler, 100 steps), C2 (128^3 curvilinear
+    TGV, 20 steps after one warm-up), NS (the C2 flow at 256^3, 2 steps after one warm-up), on all
+    cores of the process affinity, then on one thread C1 (10 steps), C2 and NS (1 step each).
+    Rates are interior nodes x steps / wall time. With gpu, the same case, dt and step count run on
+    the GPU block path (direct FP64 steps, timed with HIP events) and the two final states are
+    compared (per-field relative L2 / Linf of the primitives)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import pyoracle
+    from cfdx import workloads
     kind = "reference" if pyoracle.available("reference") else "port"
     if not pyoracle.available(kind):
         return None
     orc = pyoracle.Oracle(kind)
     cores = len(os.sched_getaffinity(0))
-    orc.set_threads(cores)
-    n = 64
-    h = 2 * math.pi / n
-    case = cfdx.periodic_case(n, n, n, grid="sinusoid", length=(2 * math.pi,) * 3, amp=0.05 * h,
-                              phases=(0.3, 1.1, 2.0), init="tgv", mach=0.1)
-    prm = pyoracle.reference_params(mach=0.1, reyl=1600.0)
-    dt = cfdx.cfl_dt(case, 0.5)
-    work = orc.new_work(case["Grid_P"].shape[0])
-    t0 = time.perf_counter()
-    orc.steps(case, 1, 0.1, dt, prm, work)
-    t1 = time.perf_counter()
-    steps = max(1, int(max_seconds / max(t1 - t0, 1e-6)) - 1)
-    steps = min(steps, 100)
-    t2 = time.perf_counter()
-    orc.steps(case, steps, 0.1, dt, prm, work)
-    t3 = time.perf_counter()
-    cups = n ** 3 * steps / (t3 - t2)
-    # the same block on one thread, a few steps
-    orc.set_threads(1)
-    t4 = time.perf_counter()
-    orc.steps(case, 2, 0.1, dt, prm, work)
-    t5 = time.perf_counter()
-    orc.set_threads(cores)
-    return {"value": cups, "unit": "cell-updates/s", "cores": cores, "kind": kind,
-            "sample": f"{steps} RK3 steps of a {n}^3 curvilinear TGV block (same kernels, FP64, OpenMP over nodes)",
-            "seconds": round(t3 - t2, 2),
-            "single_thread": {"value": n ** 3 * 2 / (t5 - t4), "cores": 1, "sample": f"2 RK3 steps of the same {n}^3 block",
-                              "seconds": round(t5 - t4, 2)}}
+    plan = {"c1": (64, 100, 0), "c2": (128, 20, 1), "ns256": (256, 2, 1)}
+    single = {"c1": 10, "c2": 1, "ns256": 1}
+    out = {}
+    for name in which:
+        n, steps, warm = plan[name]
+        t_case = time.perf_counter()
+        if name == "c1":
+            case = cfdx.periodic_case(n, n, n, **workloads.vortex_case_kw(n))
+            prm = pyoracle.reference_params(reyl=math.inf, c_mts=0.0)
+            gprm = cfdx.euler_params()
+            mach = 0.5
+        else:
+            case = cfdx.periodic_case(n, n, n, **workloads.tgv_case_kw(n))
+            prm = pyoracle.reference_params(mach=0.1, reyl=1600.0)
+            gprm = cfdx.default_params(mach=0.1, reyl=1600.0)
+            mach = 0.1
+        dt = cfdx.cfl_dt(case, 0.5)
+        work = orc.new_work(case["Grid_P"].shape[0])
+        t_case = time.perf_counter() - t_case
+        rec = {"grid": f"{n}^3", "steps": steps, "warmup": warm, "dt": dt, "host_case_s": round(t_case, 1)}
+        g = None
+        if gpu:
+            b = cfdx.Block(n, n, n)
+            b.set_case(case)
+            ctx = b.ctx
+            b.step(warm, mach, dt, gprm)
+            ctx.sync()
+            ctx.record(0)
+            b.step(steps, mach, dt, gprm)
+            ctx.record(1)
+            ctx.sync()
+            ms = ctx.elapsed_ms(0, 1)
+            g = b.get_flow()
+            b.close()
+            rec["gpu"] = {"value": n ** 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
+                          "stepping": "direct FP64 block steps"}
+        orc.set_threads(cores)
+        if warm:
+            orc.steps(case, warm, mach, dt, prm, work)
+        t0 = time.perf_counter()
+        orc.steps(case, steps, mach, dt, prm, work)
+        sec = time.perf_counter() - t0
+        rec.update({"threads": cores, "seconds": round(sec, 2), "value": n ** 3 * steps / sec,
+                    "ms_per_step": 1e3 * sec / steps})
+        if g is not None:
+            rec["gpu"]["over_cpu"] = rec["gpu"]["value"] / rec["value"]
+            rec["gpu"]["field_errors_vs_cpu"] = field_errors(g, case["FLOW"][1:])
+            del g
+        if name in single_thread:
+            orc.set_threads(1)
+            t0 = time.perf_counter()
+            orc.steps(case, single[name], mach, dt, prm, work)
+            s1 = time.perf_counter() - t0
+            rec["single_thread"] = {"steps": single[name], "seconds": round(s1, 2), "value": n ** 3 * single[name] / s1,
+                                    "ms_per_step": 1e3 * s1 / single[name]}
+        out[name] = rec
+        msg = f"bench: cpu {name} {rec['value']:.3e} cell-updates/s on {cores} threads"
+        if "single_thread" in rec:
+            msg += f", {rec['single_thread']['value']:.3e} on one"
+        if "gpu" in rec:
+            e = max(v["rel_linf"] for v in rec["gpu"]["field_errors_vs_cpu"].values())
+            msg += f"; GPU {rec['gpu']['value']:.3e}, max rel Linf vs CPU {e:.1e}"
+        print(msg, file=sys.stderr, flush=True)
+        del case, work
+    return {"kind": kind, "cores": cores, "configs": out,
+            "library": "oracle/_ref/libref.so (the reference's kernels as C, gcc -O3 -march=native -fopenmp)"
+                       if kind == "reference" else "oracle/liboracle.so (the restatement)"}
 
 
 def make_block(cfdx, cdef, precision, prm, ctx=None, host_case=None):
@@ -286,76 +361,83 @@ def dropin_path(cfdx, case, mach, dt, prm, block_value, steps: int = 3):
     value = n * steps / (ms / 1e3)
     return {"workload": "the headline case through cfdx_uq / cfdx_viscous / cfdx_weno (AoS, both faces per node)",
             "steps": steps, "value": value, "unit": "cell-updates/s", "ms_per_step": ms / steps,
-            "block_path_over_dropin": block_value / value, "state_finite": finite}
+            "block_path_over_dropin": block_value / value if block_value else None, "state_finite": finite}
 
 
-def one_gpu_share(cfdx, tune: bool, steps: int = 10, warmup: int = 2):
-    """The C2 map and flow at 256^3, plain FP64: one GPU's share of a 512^3 run split 2x2x2 (the
-    periodic wrap stands in for the neighbours' halo), set up on the device."""
-    spec, cdef, pkw, _ = workload("ns256")
-    prm = cfdx.default_params(**pkw)
+def timed_run(cfdx, name, precision, steps, warmup, tune, profile_steps=2, ctx=None, host_case=None, keep_flow=True):
+    """One workload's timed block: tuned tiles, CFL-0.5 dt from the initial state, `warmup`
+    untimed and `steps` timed RK3 steps replayed from the captured step graph (device time from
+    HIP events on the block's stream), then the per-kernel-class timing pass."""
+    spec, cdef, pkw, viscous = workload(name)
+    prm = cfdx.default_params(**pkw) if viscous else cfdx.euler_params()
     mach = cdef["kw"]["mach"]
-    tiles = tuned_tiles(cfdx, cdef, cfdx.FP64, mach, prm) if tune else None
-    b, dt = make_block(cfdx, cdef, cfdx.FP64, prm)
+    tiles = tuned_tiles(cfdx, cdef, precision, mach, prm) if tune else None
+    block, dt = make_block(cfdx, cdef, precision, prm, ctx=ctx, host_case=host_case)
     if tiles:
-        b.set_tiles(tuple(tiles["tiles"]))
-    ctx = b.ctx
-    b.step(warmup, mach, dt, prm)
-    ctx.sync()
-    ctx.record(0)
-    b.step(steps, mach, dt, prm)
-    ctx.record(1)
-    ctx.sync()
-    ms = ctx.elapsed_ms(0, 1)
-    import numpy as np
-    finite = bool(np.isfinite(cfdx_matrix(b.get_flow())).all())
-    b.close()
+        block.set_tiles(tuple(tiles["tiles"]))
+    ctx = block.ctx
+    dev_ms, tk = time_block(block, ctx, mach, dt, prm, steps, warmup, profile_steps, cfdx,
+                            tiles["tiles"] if tiles else None)
     n = cdef["ni"] * cdef["nj"] * cdef["nk"]
-    return {"workload": spec["workload"], "precision": "fp64", "steps": steps, "value": n * steps / (ms / 1e3),
-            "unit": "cell-updates/s", "ms_per_step": ms / steps, "state_finite": finite,
-            "tiles": tiles["tiles"] if tiles else None}
+    out = {"spec": spec, "cdef": cdef, "prm": prm, "mach": mach, "viscous": viscous, "dt": dt, "nodes": n,
+           "tiles": tiles, "dev_ms": dev_ms, "tk": tk, "value": n * steps / (dev_ms / 1e3),
+           "ms_per_step": dev_ms / steps, "finite": block.check_finite()}
+    if keep_flow:
+        out["flow"] = block.get_flow()
+    block.close()
+    return out
 
 
-C4_EDGES = (32, 48, 64, 96, 128)
+def per_stage_ms(tk, profile_steps):
+    return {k: v / (3 * profile_steps) for k, v in tk.items() if k.endswith("_ms")}
 
 
-def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int = 64, grid_cap: int = 0):
-    """64 periodic cubes, edge U{32,48,64,96,128}, isentropic vortex / TGV ~ Bernoulli(1/2),
-    Mach U(0.1,0.6), Re U(500,5000), set up on the device; mixed precision, one batched launch per
-    kernel class replayed from a hipGraph of one RK3 step. Also times the same solves in plain
-    FP64 one block after another."""
+def c2_leg(cfdx, steps, warmup, tune, profile_steps=2):
+    """C2 (128^3 curvilinear TGV, MTS LES) in FP64 and in mixed precision from the same initial
+    state and step count: throughput of both and per-field errors of mixed against FP64."""
+    r = {p: timed_run(cfdx, "c2", prec, steps, warmup, tune, profile_steps)
+         for p, prec in (("fp64", cfdx.FP64), ("mixed", cfdx.MIXED))}
+    out = {"workload": r["fp64"]["spec"]["workload"], "steps": steps, "warmup": warmup, "dt": r["fp64"]["dt"]}
+    for p in ("fp64", "mixed"):
+        out[p] = {"value": r[p]["value"], "unit": "cell-updates/s", "ms_per_step": r[p]["ms_per_step"],
+                  "kernel_ms_per_stage": per_stage_ms(r[p]["tk"], profile_steps),
+                  "tiles": r[p]["tiles"]["tiles"] if r[p]["tiles"] else None, "state_finite": r[p]["finite"]}
+    out["mixed_over_fp64"] = r["mixed"]["value"] / r["fp64"]["value"]
+    out["hbm_alg_frac_fp64"] = ALG_BYTES_PER_CELL_UPDATE * r["fp64"]["value"] / (HBM_PEAK_GBS * 1e9)
+    out["steps_compared"] = warmup + steps + profile_steps
+    out["field_errors_mixed_vs_fp64"] = field_errors(r["mixed"]["flow"], r["fp64"]["flow"])
+    return out
+
+
+def batched_c4(cfdx, steps: int, warmup: int, nblocks: int = 64, grid_cap: int = 0):
+    """C4: the 64 seeded periodic cubes (edge U{32,48,64,96,128}, isentropic vortex / TGV ~
+    Bernoulli(1/2), Mach U(0.1,0.6), Re U(500,5000)), set up on the device; mixed precision, one
+    batched launch per kernel class replayed from a hipGraph of one RK3 step, tuned tiles. Also
+    times the same solves in plain FP64 one block after another."""
     import numpy as np
-    rng = np.random.default_rng(seed)
+    from cfdx import tune, workloads
     ctx = cfdx.Ctx(0)
-    specs = []
-    for _ in range(nblocks):
-        e = int(rng.choice(C4_EDGES))
-        tgv = bool(rng.integers(2))
-        specs.append(dict(e=e, tgv=tgv, mach=float(rng.uniform(0.1, 0.6)), reyl=float(rng.uniform(500, 5000)),
-                          phases=tuple(float(x) for x in rng.uniform(0, 2 * math.pi, 3))))
+    specs = workloads.c4_specs(workloads.SEED, nblocks)
 
     def build(precision):
         blocks = []
         for sp in specs:
-            e = sp["e"]
-            L = 2 * math.pi if sp["tgv"] else 10.0
-            b = cfdx.Block(e, e, e, precision=precision, ctx=ctx)
-            b.init_case(grid="sinusoid", length=(L,) * 3, amp=0.05 * L / e, phases=sp["phases"],
-                        init="tgv" if sp["tgv"] else "vortex", mach=sp["mach"], center=(5.0, 5.0))
-            prm = cfdx.default_params(mach=sp["mach"], reyl=sp["reyl"])
-            b.configure(sp["mach"], 0.0, prm)
+            b = cfdx.Block(sp.edge, sp.edge, sp.edge, precision=precision, ctx=ctx)
+            b.init_case(**sp.case_kw())
+            prm = cfdx.default_params(mach=sp.mach, reyl=sp.reyl)
+            b.configure(sp.mach, 0.0, prm)
             dt = b.cfl_dt(0.5)
-            b.configure(sp["mach"], dt, prm)
-            blocks.append((b, dt, prm, sp["mach"]))
+            b.configure(sp.mach, dt, prm)
+            blocks.append((b, dt, prm, sp.mach))
         return blocks
 
-    nodes = sum(sp["e"] ** 3 for sp in specs)
+    nodes = sum(sp.edge ** 3 for sp in specs)
     blocks = build(cfdx.MIXED)
     q = cfdx.Batch(cfdx.MIXED, ctx)
     for b, *_ in blocks:
         q.add(b)
-    from cfdx import tune
-    tuned = tune.tune_batch(q, ctx, key=f"batch:c4:{seed}:{nblocks}:p{cfdx.MIXED}", steps=1, cache=tune.TileCache())
+    tuned = tune.tune_batch(q, ctx, key=f"batch:c4:{workloads.SEED}:{nblocks}:p{cfdx.MIXED}", steps=1,
+                            cache=tune.TileCache())
     if grid_cap:
         q.set_grid(grid_cap)
     q.step(warmup, True)
@@ -365,7 +447,7 @@ def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int
     ctx.record(1)
     ctx.sync()
     ms = ctx.elapsed_ms(0, 1)
-    finite = all(bool(np.isfinite(cfdx_matrix(b.get_flow())).all()) for b, *_ in blocks[:4])
+    finite = all(b.check_finite() for b, *_ in blocks)
     q.close()
     for b, *_ in blocks:
         b.close()
@@ -381,7 +463,7 @@ def batched_c4(cfdx, steps: int, warmup: int, seed: int = 20261202, nblocks: int
     ms_plain = ctx.elapsed_ms(2, 3)
     for b, *_ in plain:
         b.close()
-    hist = {e: sum(1 for sp in specs if sp["e"] == e) for e in C4_EDGES}
+    hist = {e: sum(1 for sp in specs if sp.edge == e) for e in workloads.C4_EDGES}
     return {"workload": "64 periodic cubes, edge U{32,48,64,96,128}, vortex/TGV, Mach U(0.1,0.6), Re U(500,5000)",
             "edges": hist, "nodes": nodes, "steps": steps, "precision": "mixed", "graph": True,
             "value": nodes * steps / (ms / 1e3), "unit": "cell-updates/s", "ms_per_step": ms / steps,
@@ -444,10 +526,11 @@ def airfoil_c3(cfdx, steps: int):
 SERVE_MAX_NODES = 5_000_000
 
 
-def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kind: str = "airfoil"):
-    """A bounded sample of a Poisson request stream served by continuous batching (mixed; arrival
-    order, and least work first) and by plain FP64 one request at a time, on the same stream."""
-    import numpy as np
+def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kind: str = "airfoil",
+            sjf: bool = False):
+    """A Poisson request stream (the full C5 stream: 200 requests) served by continuous batching
+    (mixed, arrival order; with sjf also least work first) and by plain FP64 one request at a
+    time, on the same stream. Latency runs from arrival to the final state on the host."""
     from cfdx import serve
     if kind == "airfoil":
         reqs = serve.airfoil_stream(nreq, rate, seed=seed, nsteps=nsteps)
@@ -458,8 +541,9 @@ def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kin
         reqs = serve.periodic_stream(nreq, rate, seed=seed, nsteps=nsteps)
         out = {"stream": f"{nreq} requests, Poisson {rate}/s, {nsteps} RK3 steps each, grids 128x64x16..384x128x32, "
                          "periodic TGV / vortex (curvilinear map), Mach U(0.1,0.6), Re U(500,5000)"}
-    runs = (("batched_mixed", cfdx.MIXED, True, "fifo"), ("batched_mixed_sjf", cfdx.MIXED, True, "sjf"),
-            ("plain_fp64", cfdx.FP64, False, "fifo"))
+    runs = [("batched_mixed", cfdx.MIXED, True, "fifo"), ("plain_fp64", cfdx.FP64, False, "fifo")]
+    if sjf:
+        runs.insert(1, ("batched_mixed_sjf", cfdx.MIXED, True, "sjf"))
     for name, prec, batched, policy in runs:
         # batched: at most SERVE_MAX_NODES nodes in flight (enough to fill the GPU; the earlier
         # requests finish first under load); "sjf" admits the queued request with the least work
@@ -470,7 +554,9 @@ def serving(cfdx, nreq: int, rate: float, nsteps: int, seed: int = 20261202, kin
         rep["peak_active"] = srv.peak_active
         rep["max_nodes_in_flight"] = srv.max_nodes
         rep["peak_bytes"] = srv.peak_bytes
-        rep["state_finite"] = all(bool(np.isfinite(cfdx_matrix(r.flow)).all()) for r in res.values())
+        # checked on the device when each request retires (Result.finite)
+        rep["state_finite"] = all(r.finite for r in res.values())
+        del res
         srv.close()
         out[name] = rep
         print(f"bench: serving {name}: p50 {rep['p50_s']:.3f}s p99 {rep['p99_s']:.3f}s "
@@ -484,20 +570,25 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="ns256", choices=["ns256", "c2", "c1", "c3"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--legs", default="compare,dropin,ns256,c4,c3,serve",
-                    help="extra legs after the headline: compare (other precision, per-field errors), dropin (the "
-                         "same case through the drop-in kernels), ns256 (256^3 FP64, one GPU's share of 512^3), c4, "
-                         "c3, serve (none: the headline only)")
+    ap.add_argument("--legs", default="compare,c2,dropin,c4,c3,serve",
+                    help="legs after the headline: compare (the other precision on the headline grid, per-field "
+                         "errors), c2 (128^3 FP64 and mixed), dropin (C2 through the drop-in kernels), c4, c3, serve "
+                         "(none: the headline only)")
+    ap.add_argument("--cpu", default="c1,c2,ns256",
+                    help="CPU-baseline configurations (all cores, then one thread; each beside the GPU on the same "
+                         "case, dt and steps), or none")
+    ap.add_argument("--no-cpu", dest="cpu", action="store_const", const="none")
+    ap.add_argument("--c2-steps", type=int, default=20)
     ap.add_argument("--c3-steps", type=int, default=500)
-    ap.add_argument("--serve-kind", default="airfoil", choices=["airfoil", "periodic"])
     ap.add_argument("--c4-steps", type=int, default=5)
-    ap.add_argument("--no-tune", dest="tune", action="store_false", help="default tiles instead of the autotuned ones")
-    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-kind", default="airfoil", choices=["airfoil", "periodic"])
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-steps", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-sjf", action="store_true", help="also serve the stream least-work-first")
+    ap.add_argument("--no-tune", dest="tune", action="store_false", help="default tiles instead of the autotuned ones")
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-kernel event timing pass")
     ap.add_argument("--copy-mib", type=int, default=256, help="size of the reference copy kernel run (HBM counter scale)")
     ap.add_argument("--sgs", default="mts", choices=["mts", "smagorinsky"],
@@ -505,17 +596,15 @@ def main() -> int:
     ap.add_argument("--c-s", dest="c_s", type=float, default=0.17, help="Smagorinsky constant")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: the library runs one GPU per process; --gpus {args.gpus} is not supported "
-              "(the periodic block is single-GPU, multi-GPU halo exchange is design-only)", file=sys.stderr)
+        print(f"bench.py: one GPU per process; --gpus {args.gpus} is not supported (the periodic block is "
+              "single-GPU; the multi-GPU halo exchange is a design only)", file=sys.stderr)
         return 2
-    legs = {x for x in args.legs.split(",") if x}
+    legs = {x for x in args.legs.split(",") if x and x != "none"}
+    cpu_cfgs = [x for x in args.cpu.split(",") if x and x != "none"]
 
-    import numpy as np
     cfdx = _import_pkg()
-    spec, cdef, pkw, viscous = workload(args.workload)
     t_setup = time.perf_counter()
-    # periodic workloads: the case built on the host (the same arrays the oracle would take)
-    case = None if "cgrid" in cdef else cfdx.periodic_case(cdef["ni"], cdef["nj"], cdef["nk"], **cdef["kw"])
+    spec, cdef, pkw, viscous = workload(args.workload)
     if args.sgs == "smagorinsky" and viscous:
         # the Smagorinsky option (no reference counterpart; the reference's own model is MTS)
         pkw = dict(pkw, sgs="smagorinsky", c_s=args.c_s)
@@ -525,10 +614,11 @@ def main() -> int:
     n = cdef["ni"] * cdef["nj"] * cdef["nk"]
     precision = cfdx.MIXED if args.precision == "mixed" else cfdx.FP64
     tiles = tuned_tiles(cfdx, cdef, precision, mach, prm) if args.tune else None
-    block, dt = make_block(cfdx, cdef, precision, prm, host_case=case)
+    block, dt = make_block(cfdx, cdef, precision, prm)
     if tiles:
         block.set_tiles(tuple(tiles["tiles"]))
-    print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+    print(f"bench: {spec['workload']} n={n} dt={dt:.4e} setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr,
+          flush=True)
 
     ctx = block.ctx
     t0 = time.perf_counter()
@@ -537,16 +627,14 @@ def main() -> int:
     wall_s = time.perf_counter() - t0
     ms_per_step = dev_ms / args.steps
     value = n * args.steps / (dev_ms / 1e3)
-    print(f"bench: {args.steps} steps device {dev_ms:.1f}ms ({wall_s:.2f}s wall incl. timing pass)", file=sys.stderr, flush=True)
-    face_launches = 9 * args.profile_steps
-    face_ms = tk["face_ms"] / face_launches
-    face_bytes = face_bytes_per_node(args.precision)
-    achieved = face_bytes * n / (face_ms / 1e3) / 1e9
-    flow = block.get_flow()
-    finite = bool(np.isfinite(cfdx_matrix(flow)).all())
+    print(f"bench: {args.steps} steps device {dev_ms:.1f}ms ({wall_s:.2f}s wall incl. timing pass) "
+          f"{value:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    finite = block.check_finite()
+    flow = block.get_flow() if "compare" in legs else None
+    block.close()
 
-    # reference copy kernel of known size (scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run);
-    # the first launch loads the code object, so time the second
+    # reference copy kernel of known size (the calibration of the FETCH_SIZE / WRITE_SIZE counters of a
+    # profiled run); the first launch loads the code object, so time the second
     nb = args.copy_mib << 20
     a, b = ctx.zeros(nb // 8), ctx.zeros(nb // 8)
     ctx.copy(b, a)
@@ -554,30 +642,66 @@ def main() -> int:
     ctx.copy(b, a)
     ctx.record(3)
     copy_ms = ctx.elapsed_ms(2, 3)
-    a.free(); b.free()
+    a.free()
+    b.free()
 
-    prof = profile_counts(cfdx, args.workload, args.precision, tiles["tiles"] if tiles else None)
-    traffic = prof["hbm_bytes"] if prof else None
-    valu = None
-    if prof:
-        sec = face_ms / 1e3
-        valu = {"wave_instr_per_launch": prof["valu_instr"],
-                "achieved_wave_instr_s": prof["valu_instr"] / sec,
-                "peak_wave_instr_s": VALU_PEAK_WAVE_INSTR_S,
-                "issue_frac": prof["valu_instr"] / sec / VALU_PEAK_WAVE_INSTR_S,
-                "fp64_tflops": prof["fp64_flop"] / sec / 1e12, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
-                "fp32_tflops": prof["fp32_flop"] / sec / 1e12,
-                "counts_from": prof["source"], "profile_rows": prof["rows"], "same_variants": prof["same_variants"],
-                "profiled_kernel_us": prof["avg_us"]}
+    # whole-step figures: the algorithmic 848 B per cell-update, the counters' measured bytes and VALU
+    # instructions per RK stage (committed profile of this workload), the static operation count
+    t_tiles = tiles["tiles"] if tiles else None
+    cf = counter_figures(cfdx, args.workload, args.precision, t_tiles, viscous, n)
+    stage_ms = per_stage_ms(tk, args.profile_steps)
+    alg_gbs = ALG_BYTES_PER_CELL_UPDATE * value / 1e9
+    step_s = ms_per_step / 1e3
+    counters = None
+    bound = None
+    if cf:
+        traffic_step = 3 * cf["stage_hbm_bytes"]
+        valu_step = 3 * cf["stage_valu_instr"]
+        hbm_meas = traffic_step / step_s / 1e9
+        valu_frac_step = valu_step / step_s / VALU_PEAK_WAVE_INSTR_S
+        per_kernel = {}
+        for k, r in cf["kernels"].items():
+            sec = r["avg_us"] * 1e-6
+            per_kernel[k] = {"avg_us_profiled": r["avg_us"], "hbm_bytes_per_launch": r["hbm_bytes"],
+                             "hbm_GBs": r["hbm_bytes"] / sec / 1e9, "hbm_frac": r["hbm_bytes"] / sec / (HBM_PEAK_GBS * 1e9),
+                             "valu_wave_instr_per_launch": r["valu_instr"],
+                             "valu_issue_frac": r["valu_instr"] / sec / VALU_PEAK_WAVE_INSTR_S,
+                             "fp64_tflops": r["fp64_flop"] / sec / 1e12, "fp32_tflops": r["fp32_flop"] / sec / 1e12}
+            per_kernel[k]["bound"] = "valu" if per_kernel[k]["valu_issue_frac"] > per_kernel[k]["hbm_frac"] else "hbm"
+        # the step's bound: the limiter of the kernels that take most of its time
+        t_by = {}
+        for k, r in cf["kernels"].items():
+            t_by[per_kernel[k]["bound"]] = t_by.get(per_kernel[k]["bound"], 0.0) + r["avg_us"]
+        bound = max(t_by, key=t_by.get)
+        counters = {"source": cf["source"], "same_variants": cf["same_variants"],
+                    "hbm_bytes_per_step": traffic_step, "hbm_bytes_per_cell_update": traffic_step / n,
+                    "traffic_over_alg": traffic_step / n / ALG_BYTES_PER_CELL_UPDATE,
+                    "measured_hbm_GBs": hbm_meas, "measured_hbm_frac": hbm_meas / HBM_PEAK_GBS,
+                    "valu_wave_instr_per_step": valu_step, "valu_issue_frac_step": valu_frac_step,
+                    "profiled_stage_us": cf["stage_profiled_us"], "kernel_time_by_bound_us": t_by,
+                    "per_kernel": per_kernel}
+    # the face sweep, timed live (HIP events around each face-kernel launch of the timing pass)
+    face_launches = 9 * args.profile_steps
+    face_ms = tk["face_ms"] / face_launches
+    fbytes = face_bytes_per_node(args.precision) * n
+    face = {"kernel": "k_face (i, j, k sweeps; mean per launch)", "avg_ms": face_ms,
+            "alg_bytes_per_launch": fbytes, "alg_GBs": fbytes / (face_ms / 1e3) / 1e9,
+            "alg_hbm_frac": fbytes / (face_ms / 1e3) / (HBM_PEAK_GBS * 1e9)}
+    if counters:
+        fk = [k for k in counters["per_kernel"] if k.startswith("k_face")]
+        face["valu_issue_frac"] = sum(counters["per_kernel"][k]["valu_wave_instr_per_launch"] for k in fk) / len(fk) \
+            / (face_ms / 1e3) / VALU_PEAK_WAVE_INSTR_S
+        face["valu_wave_instr_per_face"] = sum(counters["per_kernel"][k]["valu_wave_instr_per_launch"] for k in fk) \
+            / len(fk) * 64 / n
+        face["fp64_tflops"] = sum(cf["kernels"][k]["fp64_flop"] for k in fk) / len(fk) / (face_ms / 1e3) / 1e12
 
     counts = ctx.op_count(prm)
     ops = alg_ops_per_cell_update(counts, viscous)
-    alg = {"ops_per_cell_update": ops, "ops_per_face": counts["face"]["total"],
-           "counts": {k: v for k, v in counts.items()},
-           "achieved_tops": ops * value / 1e12, "peak_tops": FP64_PEAK_TFLOPS,
-           "frac": ops * value / (FP64_PEAK_TFLOPS * 1e12),
-           "note": "static count of the kernels' arithmetic (cfdx_op_count + the node kernels' own lines); "
-                   "divisions, square roots and reciprocals count one each"}
+    peak_ops = (FP64_PEAK_TFLOPS if precision == cfdx.FP64 else FP32_PEAK_TFLOPS) * 1e12
+    valu_alg = {"ops_per_cell_update": ops, "ops_per_face": counts["face"]["total"],
+                "achieved_tops": ops * value / 1e12, "peak_tops": peak_ops / 1e12, "frac": ops * value / peak_ops,
+                "note": "static count of the kernels' arithmetic (cfdx_op_count + the node kernels' own lines); "
+                        "divisions, square roots and reciprocals count one each"}
 
     line = {
         "metric": "cell-updates/s",
@@ -592,16 +716,19 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if precision == cfdx.FP64 else "f32 fluxes / f64 state",
         "data": "synthetic",
-        "config": dict(spec, parallelism="single GPU, one block", precision=args.precision,
+        "config": dict(spec, parallelism="single GPU, one block (one GPU's share of a 512^3 run split 2x2x2)"
+                       if args.workload == "ns256" else "single GPU, one block", precision=args.precision,
                        stepping="captured RK3-step graph replays (one-block batch)"),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_face (i, j, k sweeps; mean per launch)", "kernel_ms": face_ms,
-                     "bytes_per_launch": face_bytes * n},
-        "valu": valu,
-        "hbm_alg_frac_whole_step": ALG_BYTES_PER_CELL_UPDATE * value / (HBM_PEAK_GBS * 1e9),
-        "valu_alg_whole_step": alg,
-        "kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in tk.items() if k.endswith("_ms")},
+        "roofline": {"bound": bound or "valu", "achieved": alg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg_gbs / HBM_PEAK_GBS,
+                     "traffic": counters["hbm_bytes_per_step"] if counters else None,
+                     "achieved_is": f"{ALG_BYTES_PER_CELL_UPDATE} B algorithmic HBM bytes per cell-update x "
+                                    "cell-updates/s (whole RK3 step)",
+                     "traffic_is": "measured HBM bytes per RK3 step (FETCH_SIZE + WRITE_SIZE of the step's kernels, "
+                                   "copy-kernel calibrated)",
+                     "counters": counters, "face_kernel": face},
+        "valu_alg_whole_step": valu_alg,
+        "kernel_ms_per_stage": stage_ms,
         "state_finite": finite,
         "copy_kernel": {"bytes_moved": 2 * nb, "ms": copy_ms, "GBs": 2 * nb / (copy_ms / 1e3) / 1e9},
         "tiles": ({"face_i_j_k_node": tiles["tiles"], "cached": tiles["cached"], "times_ms": tiles.get("times_ms")}
@@ -609,37 +736,36 @@ def main() -> int:
     }
 
     if "compare" in legs:
-        # the other precision on the same initial state and step count: throughput and per-field error
+        # the other precision on the same grid, initial state and step count: throughput and per-field error
         other = cfdx.FP64 if precision == cfdx.MIXED else cfdx.MIXED
-        otiles = tuned_tiles(cfdx, cdef, other, mach, prm) if args.tune else None
-        ob, _ = make_block(cfdx, cdef, other, prm, ctx=ctx, host_case=case)
-        ob.configure(mach, dt, prm)
-        if otiles:
-            ob.set_tiles(tuple(otiles["tiles"]))
-        o_ms, o_tk = time_block(ob, ctx, mach, dt, prm, args.steps, args.warmup, args.profile_steps, cfdx,
-                                otiles["tiles"] if otiles else None)
-        oflow = ob.get_flow()
-        ob.close()
-        mixed_flow, fp64_flow = (oflow, flow) if other == cfdx.MIXED else (flow, oflow)
-        o_value = n * args.steps / (o_ms / 1e3)
+        o = timed_run(cfdx, args.workload, other, args.steps, args.warmup, args.tune, args.profile_steps, ctx=ctx)
+        mixed_flow, fp64_flow = (o["flow"], flow) if other == cfdx.MIXED else (flow, o["flow"])
         line["mixed_vs_fp64"] = {
             "other_precision": "mixed" if other == cfdx.MIXED else "fp64",
-            "other_value": o_value,
-            "mixed_over_fp64": (o_value / value) if other == cfdx.MIXED else (value / o_value),
-            "other_kernel_ms_per_stage": {k: v / (3 * args.profile_steps) for k, v in o_tk.items() if k.endswith("_ms")},
+            "other_value": o["value"], "other_ms_per_step": o["ms_per_step"],
+            "mixed_over_fp64": (o["value"] / value) if other == cfdx.MIXED else (value / o["value"]),
+            "other_kernel_ms_per_stage": per_stage_ms(o["tk"], args.profile_steps),
+            "other_tiles": o["tiles"]["tiles"] if o["tiles"] else None,
             "steps_compared": args.warmup + args.steps + args.profile_steps,
             "field_errors": field_errors(mixed_flow, fp64_flow),
         }
-        print(f"bench: other precision {o_value:.3e} cell-updates/s", file=sys.stderr, flush=True)
-    if "dropin" in legs and "cgrid" not in cdef:
-        line["dropin_path"] = dropin_path(cfdx, case, mach, dt, prm, value)
+        del o, mixed_flow, fp64_flow
+        print(f"bench: other precision {line['mixed_vs_fp64']['other_value']:.3e} cell-updates/s", file=sys.stderr,
+              flush=True)
+    del flow
+    if "c2" in legs:
+        line["c2"] = c2_leg(cfdx, args.c2_steps, args.warmup, args.tune, args.profile_steps)
+        print(f"bench: c2 fp64 {line['c2']['fp64']['value']:.3e} mixed {line['c2']['mixed']['value']:.3e}",
+              file=sys.stderr, flush=True)
+    if "dropin" in legs:
+        _, c2def, c2kw, _ = workload("c2")
+        case = cfdx.periodic_case(128, 128, 128, **c2def["kw"])
+        c2prm = cfdx.default_params(**c2kw)
+        c2dt = cfdx.cfl_dt(case, 0.5, c2prm)
+        c2val = line["c2"]["fp64"]["value"] if "c2" in line else None
+        line["dropin_path"] = dropin_path(cfdx, case, 0.1, c2dt, c2prm, c2val)
+        del case
         print(f"bench: drop-in kernels {line['dropin_path']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
-    del case
-    block.close()
-
-    if "ns256" in legs:
-        line["one_gpu_share_256"] = one_gpu_share(cfdx, args.tune)
-        print(f"bench: 256^3 {line['one_gpu_share_256']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
     if "c4" in legs:
         line["batched_c4"] = batched_c4(cfdx, args.c4_steps, 1)
         print(f"bench: c4 {line['batched_c4']['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
@@ -648,13 +774,28 @@ def main() -> int:
         print(f"bench: c3 mixed {line['airfoil_c3']['mixed_value']:.3e} fp64 {line['airfoil_c3']['fp64_value']:.3e}",
               file=sys.stderr, flush=True)
     if "serve" in legs:
-        line["serving"] = serving(cfdx, args.serve_requests, args.serve_rate, args.serve_steps, kind=args.serve_kind)
+        line["serving"] = serving(cfdx, args.serve_requests, args.serve_rate, args.serve_steps, kind=args.serve_kind,
+                                  sjf=args.serve_sjf)
 
-    if not args.no_cpu:
-        cb = cpu_baseline(cfdx)
+    if cpu_cfgs:
+        cb = cpu_baseline(cfdx, tuple(cpu_cfgs), tuple(cpu_cfgs))
         if cb:
-            line["cpu_baseline"] = cb
-            line["gpu_over_cpu"] = value / cb["value"]
+            # the headline workload's all-core rate is the baseline value; every configuration in "configs"
+            head = cb["configs"].get(args.workload)
+            key = args.workload if head else next(iter(cb["configs"]))
+            rec = cb["configs"][key]
+            line["cpu_baseline"] = {
+                "value": rec["value"], "unit": "cell-updates/s", "cores": rec["threads"], "kind": cb["kind"],
+                "sample": f"{rec['steps']} RK3 steps of {key} ({rec['grid']}) after {rec['warmup']} warm-up, the "
+                          f"reference's stage sequence over its AoS arrays, OpenMP over nodes on all "
+                          f"{rec['threads']} cores of the process affinity",
+                "library": cb["library"], "configs": cb["configs"]}
+            line["gpu_over_cpu"] = value / rec["value"]
+            gpu_rates = {args.workload: value}
+            if "c2" in line:
+                gpu_rates["c2"] = line["c2"]["fp64"]["value"]
+            line["cpu_baseline"]["gpu_over_cpu_by_config"] = {k: gpu_rates[k] / cb["configs"][k]["value"]
+                                                              for k in gpu_rates if k in cb["configs"]}
     print(json.dumps(line))
     return 0
 

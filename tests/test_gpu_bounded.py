@@ -161,3 +161,32 @@ def test_geometry_not_finite_refused(ctx):
     assert e.value.code == cfdx.E_NONFINITE
     b.set_geometry_soa(geo)                       # a finite geometry is taken afterwards
     b.close()
+
+
+def test_cgrid_trailing_edge_follows_the_tip_rule(ctx):
+    """The reference holds its tip nodes (Solver.cpp:660-680) and its wall nodes at singular
+    points of unstructured meshes (Flow_Initialisation.cpp:1194-1273) at rest with p and T of
+    the neighbour off the wall and rho = 1.4 M^2 p / T. On the C-grid the trailing-edge nodes
+    (j = 0, i = cut_i0 and cut_i1, where the wall range meets the wake cut) are ordinary wall
+    nodes of the block's j- side: after RK3 steps from a perturbed state they obey the same rule,
+    and the wake-cut row j = 0 next to them holds one state on both sides of the cut."""
+    d, xyz, geo, mach, aoa, kw = cgrid_case()
+    prm = cfdx.default_params(**kw)
+    b, _ = run_block(ctx, d, geo, perturbed(xyz[cfdx.owned_index(d)], mach, aoa), mach, None, prm, 4)
+    f = b.get_flow()
+    b.close()
+    ni, nj, nk = d.ni, d.nj, d.nk
+    at = lambda i, j, k: i + ni * (j + nj * k)
+    M2 = 1.4 * mach * mach
+    for k in range(nk):
+        for i in (d.cut_i0, d.cut_i1):
+            te, up = f[at(i, 0, k)], f[at(i, 1, k)]
+            assert te["u"] == 0.0 and te["v"] == 0.0 and te["w"] == 0.0, (i, k)
+            assert abs(te["p"] - up["p"]) <= 1e-12 * up["p"], (i, k)
+            T_up = M2 * up["p"] / up["rho"]
+            assert abs(M2 * te["p"] / te["rho"] - T_up) <= 1e-12 * T_up, (i, k)
+        # the wake cut beside the trailing edge: node (i, 0) of the lower half is node (ni-1-i, 0)
+        for i in (d.cut_i0 - 1, d.cut_i0 - 2):
+            lo, hi = f[at(i, 0, k)], f[at(ni - 1 - i, 0, k)]
+            for c in FIELDS:
+                assert lo[c] == hi[c], (i, k, c)
