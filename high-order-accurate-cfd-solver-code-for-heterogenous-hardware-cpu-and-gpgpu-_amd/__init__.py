@@ -140,6 +140,7 @@ EXPORTS = {
     "cfdx_batch_set_grid": (C.c_int, [_vp, C.c_uint32]),
     "cfdx_batch_step": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cfdx_block_rhs": (C.c_int, [_vp, C.POINTER(Params), _vp]),
+    "cfdx_block_rhs_noalloc": (C.c_int, [_vp, C.POINTER(Params), _vp]),
     "cfdx_block_set_timing": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_timing": (C.c_int, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(C.c_int)]),
     "cfdx_block_timing_split": (C.c_int, [_vp, C.POINTER(_d)]),
@@ -663,6 +664,13 @@ class Block:
         out = np.zeros((5, self.n))
         _check(_lib.cfdx_block_rhs(self.h, C.byref(params), _ptr(out)))
         return out
+
+    def rhs_noalloc(self, params: Params, out: "DeviceArray") -> None:
+        """The right-hand side into the device array `out` (5 x owned nodes doubles) through the
+        entry point that allocates nothing; asynchronous on the context's stream."""
+        if out.nbytes < 5 * self.n * 8:
+            raise ValueError(f"rhs_noalloc: out holds {out.nbytes} bytes, {5 * self.n * 8} needed")
+        _check(_lib.cfdx_block_rhs_noalloc(self.h, C.byref(params), _vp(out.ptr)))
 
     def set_timing(self, on: bool) -> None:
         _check(_lib.cfdx_block_set_timing(self.h, int(on)))

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "block.hpp"
@@ -714,6 +715,88 @@ extern "C" int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *
     b->B.Lout = nullptr;
     ++b->version;
     (void)hipFree(Ld);
+    return rc;
+}
+
+// ---- right-hand side into device memory, with no allocation ----------------------------------
+// The stored-node right-hand side of cfdx_block_rhs_noalloc lives in one fixed array per device
+// (64 MiB: 5 values of at most RHS_SCRATCH_NODES stored nodes), held from the library's load to
+// the end of the process. Host code never passes the array's name to a kernel: its device
+// address is resolved once per device by cfdx::scratch_resolve (from cfdx_ctx_create, with the
+// device current) and kept in scratch_dev.
+constexpr size_t RHS_SCRATCH_DOUBLES = (size_t)8 << 20;
+constexpr long long RHS_SCRATCH_NODES = (long long)(RHS_SCRATCH_DOUBLES / 5);
+static __device__ __align__(256) double scratch[RHS_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+double *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> g(scratch_mu);
+    return (device >= 0 && device < SCRATCH_MAX_DEVICES) ? scratch_dev[device] : nullptr;
+}
+
+// owned-node values (owned order, component-major with stride ni nj nk) from the stored-node
+// array Ls (component-major with stride n)
+__global__ void k_gather_owned(const double *__restrict__ Ls, long long n, double *__restrict__ Lo, int ni, int nj,
+                               int nk, int o0, int o1, int o2, int pi, int pj)
+{
+    const long long no = (long long)ni * nj * nk;
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= no) return;
+    const int i = (int)(o % ni);
+    const long long r = o / ni;
+    const int j = (int)(r % nj), k = (int)(r / nj);
+    const size_t s = (size_t)(i + o0) + (size_t)pi * ((size_t)(j + o1) + (size_t)pj * (size_t)(k + o2));
+    for (int c = 0; c < 5; ++c) Lo[(size_t)c * no + o] = Ls[(size_t)c * n + s];
+}
+}  // namespace
+
+int cfdx::scratch_resolve(int device)
+{
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+        return cfdx::fail(CFDX_E_UNSUPPORTED, "device %d: the RHS scratch table holds devices 0..%d", device,
+                          SCRATCH_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (scratch_dev[device]) return CFDX_OK;
+    void *p = nullptr;
+    CFDX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    scratch_dev[device] = (double *)p;
+    return CFDX_OK;
+}
+
+extern "C" int cfdx_block_rhs_noalloc(cfdx_block *b, const cfdx_params *params, double *L_dev)
+{
+    if (!b || !L_dev) return cfdx::fail(CFDX_E_INVALID, "block_rhs_noalloc: null argument");
+    CFDX_TRY(check_params(params));
+    const long long n = b->B.n;
+    if (n > RHS_SCRATCH_NODES)
+        return cfdx::fail(CFDX_E_CAPACITY,
+                          "block_rhs_noalloc: %lld stored nodes exceed the %lld of the fixed 64 MiB scratch array "
+                          "(cfdx_block_rhs has no limit)", n, RHS_SCRATCH_NODES);
+    CFDX_TRY(enter_block(b));
+    double *Ls = scratch_of(b->ctx->device);
+    if (!Ls) {   // not resolved when the context was made: resolve now (the device is current)
+        CFDX_TRY(cfdx::scratch_resolve(b->ctx->device));
+        Ls = scratch_of(b->ctx->device);
+    }
+    configure(b, b->last_mach, b->B.del_t, params);
+    b->B.Lout = Ls;
+    ++b->version;
+    int rc = upload_single(b);
+    if (rc == CFDX_OK) rc = run_stage(b, 3);
+    if (rc == CFDX_OK) {
+        const long long no = (long long)b->desc.ni * b->desc.nj * b->desc.nk;
+        hipLaunchKernelGGL(k_gather_owned, dim3(cfdx::blocks_for(no, 256)), dim3(256), 0, b->ctx->stream, Ls, n, L_dev,
+                           b->desc.ni, b->desc.nj, b->desc.nk, b->B.off[0], b->B.off[1], b->B.off[2], b->B.pi,
+                           b->B.pj);
+        rc = cfdx::check_launch("gather_owned");
+    }
+    b->B.Lout = nullptr;
+    ++b->version;
     return rc;
 }
 

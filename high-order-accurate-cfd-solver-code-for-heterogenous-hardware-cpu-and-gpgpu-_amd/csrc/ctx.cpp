@@ -103,6 +103,9 @@ extern "C" int cfdx_ctx_create(int device, void *stream, cfdx_ctx **out)
     CFDX_HIP(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
     if (device < 0 || device >= ndev) return cfdx::fail(CFDX_E_INVALID, "ctx_create: device %d of %d", device, ndev);
     CFDX_HIP(hipSetDevice(device), "hipSetDevice");
+    // the device address of the no-allocation RHS's fixed array; a failure here is reported by
+    // that entry point (which resolves again), not by every context
+    (void)cfdx::scratch_resolve(device);
     cfdx_ctx *c = new cfdx_ctx;
     c->device = device;
     if (stream) {

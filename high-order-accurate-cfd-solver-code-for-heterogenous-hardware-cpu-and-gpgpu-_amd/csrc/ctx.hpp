@@ -27,6 +27,9 @@ struct cfdx_ctx {
 
 namespace cfdx {
 void ctx_retain(cfdx_ctx *ctx);
+// Resolves the device address of the fixed RHS scratch array on `device`, which must be the
+// current device (once per device; later calls return at once).
+int scratch_resolve(int device);
 int ctx_release(cfdx_ctx *ctx);
 
 // Device memory for blocks: a cached allocation of at least `bytes` (at most a quarter more;

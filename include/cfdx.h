@@ -253,6 +253,16 @@ int cfdx_block_step(cfdx_block *b, double Mach, double del_t, int nsteps, const 
  * (component-major: L[c*n + node]) copied to the host. Synchronous. */
 int cfdx_block_rhs(cfdx_block *b, const cfdx_params *params, double *L_host);
 
+/* cfdx_block_rhs with no allocation: the same L (same kernels, bitwise) written to DEVICE memory
+ * L_dev (5 * ni*nj*nk doubles, component-major over the owned nodes), asynchronous on the context
+ * stream (L_dev is ready once the stream reaches it; cfdx_sync). No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under stream capture, serving
+ * loops, harnesses that forbid hipMalloc). The stored-node L is kept in a fixed 64 MiB array of
+ * each device, so blocks of more than 1,677,721 stored nodes are refused (CFDX_E_CAPACITY). Every
+ * call on one device shares that array: calls that overlap on one GPU (other streams, other
+ * threads) race. Calls no hipMalloc / hipFree. */
+int cfdx_block_rhs_noalloc(cfdx_block *b, const cfdx_params *params, double *L_dev);
+
 /* Device time in ms of the kernels of the last cfdx_block_step, per kernel class,
  * measured with HIP events on the context stream when timing is on. */
 int cfdx_block_set_timing(cfdx_block *b, int on);
