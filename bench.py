@@ -323,6 +323,7 @@ def time_block(block, ctx, mach, dt, prm, steps, warmup, profile_steps, cfdx=Non
     block.set_timing(True)
     block.step(profile_steps, mach, dt, prm)
     tk = block.timing()
+    tk["split_ms"] = block.timing_split()   # viscous, face i, face j, face k, update
     block.set_timing(False)
     return dev_ms, tk
 
@@ -389,7 +390,11 @@ def timed_run(cfdx, name, precision, steps, warmup, tune, profile_steps=2, ctx=N
 
 
 def per_stage_ms(tk, profile_steps):
-    return {k: v / (3 * profile_steps) for k, v in tk.items() if k.endswith("_ms")}
+    out = {k: v / (3 * profile_steps) for k, v in tk.items() if k.endswith("_ms") and not isinstance(v, list)}
+    if "split_ms" in tk:
+        names = ("visc", "face_i", "face_j", "face_k", "update")
+        out["split_ms"] = {n: v / (3 * profile_steps) for n, v in zip(names, tk["split_ms"])}
+    return out
 
 
 def c2_leg(cfdx, steps, warmup, tune, profile_steps=2):
@@ -468,6 +473,10 @@ def batched_c4(cfdx, steps: int, warmup: int, nblocks: int = 64, grid_cap: int =
             "edges": hist, "nodes": nodes, "steps": steps, "precision": "mixed", "graph": True,
             "value": nodes * steps / (ms / 1e3), "unit": "cell-updates/s", "ms_per_step": ms / steps,
             "tiles": {"face_i_j_k_node": tuned["tiles"], "grid_cap": tuned["grid_cap"], "cached": tuned["cached"]},
+            # one RK3 step per grid size (0: one workgroup per tile; else a persistent grid of that many
+            # workgroups striding over the tiles), timed by the tuner
+            "grid_cap_step_ms": (dict(zip(map(str, tune.GRID_CAPS), tuned["times_ms"]["grid_cap"]))
+                                 if tuned.get("times_ms") and "grid_cap" in tuned["times_ms"] else None),
             "plain_fp64_sequential": nodes * steps / (ms_plain / 1e3),
             "speedup_vs_plain_fp64": ms_plain / ms, "state_finite": finite}
 
