@@ -67,14 +67,16 @@ class CaseDesc(C.Structure):
 
 
 class Tiles(C.Structure):
-    _fields_ = [("face", C.c_int32 * 3), ("node", C.c_int32)]
+    """cfdx_tiles: (face i, face j, face k, node) variant indices and the node-tile strip (rows of
+    node tiles per strip walked plane by plane; 0: whole planes)."""
+    _fields_ = [("face", C.c_int32 * 3), ("node", C.c_int32), ("strip", C.c_int32)]
 
     def as_tuple(self) -> tuple:
-        return (self.face[0], self.face[1], self.face[2], self.node)
+        return (self.face[0], self.face[1], self.face[2], self.node, self.strip)
 
     @classmethod
     def of(cls, t) -> "Tiles":
-        return cls((C.c_int32 * 3)(t[0], t[1], t[2]), t[3])
+        return cls((C.c_int32 * 3)(t[0], t[1], t[2]), t[3], t[4] if len(t) > 4 else 0)
 
 
 class CfdxError(RuntimeError):

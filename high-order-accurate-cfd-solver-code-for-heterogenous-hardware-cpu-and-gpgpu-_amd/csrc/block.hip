@@ -229,6 +229,7 @@ int upload_single(cfdx_block *b)
 {
     if (b->uploaded == b->version) return CFDX_OK;
     BlockDev B = b->B;
+    B.strip = (uint32_t)b->tiles.strip;
     uint32_t pre[8];
     for (int kind = 0; kind < 4; ++kind) {
         B.tile0[kind] = 0;
@@ -332,6 +333,7 @@ int upload_batch(cfdx_batch *q)
     q->any_visc = false;
     for (size_t i = 0; i < nb; ++i) {
         h[i] = q->blocks[i]->B;
+        h[i].strip = (uint32_t)q->tiles.strip;
         for (int kind = 0; kind < 4; ++kind) {
             h[i].tile0[kind] = tot[kind];
             pre[kind * (q->cap + 1) + i] = tot[kind];
@@ -445,6 +447,7 @@ extern "C" int cfdx_block_create(cfdx_ctx *ctx, const cfdx_block_desc *desc, cfd
         p += (size_t)22 * n * sizeof(float);
     }
     B.Lout = nullptr;
+    B.strip = 0;
     B.code = nullptr;
     if (desc->periodic != 7) {
         const std::vector<uint8_t> codes = loc_codes(desc);
@@ -872,8 +875,10 @@ int to_cfg(const cfdx_tiles *t, cfdx::blk::TileCfg &c)
     for (int d = 1; d < 3; ++d)
         if (t->face[d] < 0 || t->face[d] >= cfdx::blk::N_FACE_JK) return cfdx::fail(CFDX_E_INVALID, "tiles: face[%d] variant %d out of range", d, t->face[d]);
     if (t->node < 0 || t->node >= cfdx::blk::N_NODE) return cfdx::fail(CFDX_E_INVALID, "tiles: node variant %d out of range", t->node);
+    if (t->strip < 0 || t->strip > 4096) return cfdx::fail(CFDX_E_INVALID, "tiles: strip %d out of 0 .. 4096", t->strip);
     for (int d = 0; d < 3; ++d) c.face[d] = t->face[d];
     c.node = t->node;
+    c.strip = t->strip;
     return CFDX_OK;
 }
 }  // namespace
@@ -904,6 +909,7 @@ extern "C" int cfdx_block_get_tiles(cfdx_block *b, cfdx_tiles *t)
     if (!b || !t) return cfdx::fail(CFDX_E_INVALID, "block_get_tiles: null argument");
     for (int d = 0; d < 3; ++d) t->face[d] = b->tiles.face[d];
     t->node = b->tiles.node;
+    t->strip = b->tiles.strip;
     return CFDX_OK;
 }
 

@@ -23,9 +23,11 @@ constexpr int NODE_TILES[N_NODE][2] = {{64, 4}, {32, 8}, {128, 2}, {256, 1}};
 struct TileCfg {
     int face[3] = {0, 0, 0};   // variant of the i, j, k face sweeps
     int node = 0;              // variant of the node kernels
+    int strip = 0;             // node tile order: strips of this many tile rows (0: whole planes)
     bool operator==(const TileCfg &o) const
     {
-        return face[0] == o.face[0] && face[1] == o.face[1] && face[2] == o.face[2] && node == o.node;
+        return face[0] == o.face[0] && face[1] == o.face[1] && face[2] == o.face[2] && node == o.node &&
+               strip == o.strip;
     }
 };
 
@@ -41,6 +43,7 @@ struct BlockDev {
     int cut_i0, cut_i1;         // wall range of a CFDX_BC_WALL_CUT side
     double cos_aoa, sin_aoa;    // far-field flow direction
     uint32_t tile0[4];          // first global tile of this block: node tiles, face tiles x, y, z
+    uint32_t strip;             // node tiles: rows of tiles per strip walked plane by plane (0: whole planes)
     const double *det;          // det at the node
     const double *m[9];         // node metrics zeta_x .. xi_z
     const double *kf[3][3];     // face metric row of direction d at the +1/2 face (zeta_?ip, eta_?jp, xi_?kp)

@@ -168,6 +168,29 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb)
     return x * q + (x < r ? x : r) + l;
 }
 
+// Node tile lt of a block: (i0, j0) of its first node and its plane k. Tiles run along i, then
+// (with B.strip = S > 0) through strips of S tile rows, each strip plane by plane, so that the
+// k +- 2 planes of a strip are recent in the L2 of the XCD that walks it; S = 0: plane by plane.
+__device__ __forceinline__ void node_tile(const BlockDev &B, uint32_t lt, int TX, int TY, int &i0, int &j0, int &k)
+{
+    const uint32_t ntx = (B.ni + TX - 1) / TX, nty = (B.nj + TY - 1) / TY, nk = (uint32_t)B.nk;
+    const uint32_t S = B.strip;
+    if (S == 0 || S >= nty) {
+        i0 = (int)((lt % ntx) * TX);
+        j0 = (int)(((lt / ntx) % nty) * TY);
+        k = (int)(lt / (ntx * nty));
+        return;
+    }
+    const uint32_t per_full = ntx * S * nk, full = nty / S;
+    uint32_t sidx, r, rows;
+    if (lt < full * per_full) { sidx = lt / per_full; r = lt % per_full; rows = S; }
+    else { sidx = full; r = lt - full * per_full; rows = nty - full * S; }
+    k = (int)(r / (ntx * rows));
+    const uint32_t q = r % (ntx * rows);
+    i0 = (int)((q % ntx) * TX);
+    j0 = (int)((sidx * S + q / ntx) * TY);
+}
+
 // block of a global tile id: the largest b with prefix[b] <= t (prefix[0] = 0, prefix[nblk] = total)
 __device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ prefix, uint32_t nblk, uint32_t t)
 {
@@ -184,15 +207,22 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t *__restrict__ pref
 // ---------------------------------------------------------------------------
 // Fv, Ev, Gv (+ mu) of one node. LOC: a shell node of a bounded block (loc-selected forms near
 // the sides); otherwise the central forms.
-template <typename S, bool LOC, bool NT>
-__device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k, int mode)
+// PT: the stage vector of the node's in-plane stencil (i and j offsets -2 .. 2) is read from a
+// plane tile in LDS: pt[(c * rows + ty + 2 + dj) * cols + tx + 2 + di] (the same doubles the global
+// loads would give); the k offsets are read from global memory.
+template <typename S, bool LOC, bool NT, bool PT = false>
+__device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k, int mode, const double *pt = nullptr,
+                                          int tx = 0, int ty = 0, int cols = 0, int rows = 0)
 {
     const uint32_t g = node_index(B, i, j, k);
     // stage vector into the viscous kernel: U0 at j = 0, the raw U1 / U2 after (Solver.cpp:427-435)
     const double *const *Uin = (mode == 0 || mode == 3) ? B.U0 : B.Us;
+    auto tile = [&](int c, int d, int o) -> double {
+        return pt[(c * rows + ty + 2 + (d == 1 ? o : 0)) * cols + tx + 2 + (d == 0 ? o : 0)];
+    };
     double U[5];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) U[c] = ld(Uin[c], g);
+    for (int c = 0; c < 5; ++c) U[c] = PT ? tile(c, 0, 0) : ld(Uin[c], g);
     double dU[5][3], KS[3];
     const double iU0 = 1.0 / U[0];
 #pragma unroll
@@ -212,6 +242,13 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
                 up1[c] = ld(u, p1);
                 const double f[7] = {ld(u, m3), ld(u, m2), um1[c], U[c], up1[c], ld(u, p2), ld(u, p3)};
                 dU[c][d] = loc_derivative(f, LI, LM2, LM1, LP1, LP2, false);
+            }
+        } else if (PT && d < 2) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                um1[c] = tile(c, d, -1);
+                up1[c] = tile(c, d, 1);
+                dU[c][d] = (1.0 / 12.0) * (8.0 * (up1[c] - um1[c]) - (tile(c, d, 2) - tile(c, d, -2)));
             }
         } else {
 #pragma unroll
@@ -257,10 +294,15 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
 // Node tiles of the list first (tile ids through the XCD remap), then, for lists with bounded
 // blocks, nblk x shell_tiles shell tiles of TX * TY shell nodes each (block = id / shell_tiles),
 // dispatched last in plain order so that they spread over the XCDs at the end of the launch.
+// Tiles of at least 4 rows stage the in-plane stencil states of the tile in LDS (PLANE): each
+// state is loaded once per tile instead of once per node that reaches it.
 template <typename S, int TX, int TY, bool BND>
 __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                                            uint32_t nblk, uint32_t total, int mode, uint32_t shell_tiles)
 {
+    constexpr bool PLANE = TY >= 4;
+    constexpr int PC = TX + 4, PR = TY + 4;
+    __shared__ double plane[PLANE ? 5 * PR * PC : 1];
     const uint32_t all = total + (BND ? nblk * shell_tiles : 0u);
     for (uint32_t wg = blockIdx.x; wg < all; wg += gridDim.x) {
         if (BND && wg >= total) {
@@ -277,13 +319,30 @@ __global__ void __launch_bounds__(TX *TY) k_visc(const BlockDev *__restrict__ bl
         const BlockDev &B = blks[find_block(prefix, nblk, t)];
         if (!B.viscous) continue;
         const uint32_t lt = t - B.tile0[0];
-        const uint32_t ntx = (B.ni + TX - 1) / TX, nty = (B.nj + TY - 1) / TY;
-        const int i = (int)((lt % ntx) * TX + threadIdx.x % TX);
-        const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
-        const int k = (int)(lt / (ntx * nty));
+        int i0, j0, k;
+        node_tile(B, lt, TX, TY, i0, j0, k);
+        const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+        const int i = i0 + tx, j = j0 + ty;
+        if constexpr (PLANE) {
+            // the stage vector of the tile and its in-plane halo (2 nodes on each side, corners
+            // left out) into LDS; points past the last node a tile node reaches are skipped
+            const double *const *Uin = (mode == 0 || mode == 3) ? B.U0 : B.Us;
+            __syncthreads();   // the previous tile's plane fully read
+            for (int p = (int)threadIdx.x; p < PR * PC; p += TX * TY) {
+                const int r = p / PC, q = p % PC;
+                const bool hi = q >= 2 && q < TX + 2, hj = r >= 2 && r < TY + 2;
+                if (!hi && !hj) continue;
+                if (i0 + q - 2 > B.ni + 1 || j0 + r - 2 > B.nj + 1) continue;
+                const uint32_t x = node_index(B, along_shift(B, 0, i0, q - 2, B.ni), along_shift(B, 1, j0, r - 2, B.nj), k);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) plane[(c * PR + r) * PC + q] = ld(Uin[c], x);
+            }
+            __syncthreads();
+        }
         if (i >= B.ni || j >= B.nj) continue;
         if (BND && B.code && in_shell(B, i, j, k)) continue;   // a shell tile's node
-        visc_node<S, false, !BND>(B, i, j, k, mode);
+        if constexpr (PLANE) visc_node<S, false, !BND, true>(B, i, j, k, mode, plane, tx, ty, PC, PR);
+        else visc_node<S, false, !BND>(B, i, j, k, mode);
     }
 }
 
@@ -553,10 +612,9 @@ __global__ void __launch_bounds__(TX *TY) k_update(const BlockDev *__restrict__ 
         const uint32_t t = xcd_tile(wg, total);
         const BlockDev &B = blks[find_block(prefix, nblk, t)];
         const uint32_t lt = t - B.tile0[0];
-        const uint32_t ntx = (B.ni + TX - 1) / TX, nty = (B.nj + TY - 1) / TY;
-        const int i = (int)((lt % ntx) * TX + threadIdx.x % TX);
-        const int j = (int)(((lt / ntx) % nty) * TY + threadIdx.x / TX);
-        const int k = (int)(lt / (ntx * nty));
+        int i0, j0, k;
+        node_tile(B, lt, TX, TY, i0, j0, k);
+        const int i = i0 + (int)(threadIdx.x % TX), j = j0 + (int)(threadIdx.x / TX);
         if (i < B.ni && j < B.nj && !(BND && B.code && in_shell(B, i, j, k)))
             update_node<S, false, !BND>(B, i, j, k, mode, mdu, mdv, mde);
         if (mode == 2 && B.mon) monitor_wave(B, lt * (TX * TY / 64) + threadIdx.x / 64, mdu, mdv, mde);

@@ -20,6 +20,8 @@ from . import Batch, Block, Ctx, tile_variants
 
 DEFAULT_CACHE = os.environ.get("CFDX_TILE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "cfdx", "tiles.json"))
 GRID_CAPS = (0, 1024, 2048, 4096, 8192)
+# node-tile orders tried: strips of this many tile rows walked plane by plane (0: whole planes)
+STRIPS = (0, 4, 8, 16)
 
 
 class TileCache:
@@ -54,7 +56,7 @@ class TileCache:
 def tables_tag() -> str:
     """The compiled tile shapes, so that a cached choice is dropped when the library's tables change."""
     shapes = [tile_variants(kind) for kind in range(3)]
-    return "t" + "-".join("_".join(f"{a}x{b}" for a, b in kind) for kind in shapes)
+    return "t" + "-".join("_".join(f"{a}x{b}" for a, b in kind) for kind in shapes) + "-s" + "_".join(map(str, STRIPS))
 
 
 def block_key(ni: int, nj: int, nk: int, precision: int, viscous: bool, periodic: int = 7) -> str:
@@ -68,6 +70,15 @@ def _counts() -> list[int]:
     return [len(tile_variants(0)), len(tile_variants(1)), len(tile_variants(1)), len(tile_variants(2))]
 
 
+def _options() -> list[tuple]:
+    """Per tuned class, the values tried: variant indices of the face sweeps and node kernels, then
+    the node-tile strips."""
+    return [tuple(range(n)) for n in _counts()] + [STRIPS]
+
+
+NAMES = ("face_i", "face_j", "face_k", "node", "strip")
+
+
 def tune_block(block: Block, mach: float, dt: float, params, steps: int = 2, cache: TileCache | None = None,
                force: bool = False) -> dict:
     """Chooses and sets the tile variants of one block; returns {"tiles", "times_ms", "cached"}."""
@@ -79,10 +90,9 @@ def tune_block(block: Block, mach: float, dt: float, params, steps: int = 2, cac
             return dict(hit, cached=True)
     best = list(block.get_tiles())
     times: dict[str, list[float]] = {}
-    names = ("face_i", "face_j", "face_k", "node")
-    for cls, count in enumerate(_counts()):
+    for cls, opts in enumerate(_options()):
         t_cls = []
-        for v in range(count):
+        for v in opts:
             cfg = list(best)
             cfg[cls] = v
             block.set_tiles(tuple(cfg))
@@ -92,8 +102,8 @@ def tune_block(block: Block, mach: float, dt: float, params, steps: int = 2, cac
             ms = block.timing_split()
             block.set_timing(False)
             t_cls.append(ms[1 + cls] if cls < 3 else ms[0] + ms[4])
-        times[names[cls]] = t_cls
-        best[cls] = min(range(count), key=lambda v: t_cls[v])
+        times[NAMES[cls]] = t_cls
+        best[cls] = opts[min(range(len(opts)), key=lambda v: t_cls[v])]
     block.set_tiles(tuple(best))
     out = {"tiles": best, "times_ms": times}
     if cache is not None:
@@ -125,16 +135,16 @@ def tune_batch(batch: Batch, ctx: Ctx, key: str | None = None, steps: int = 2, c
         ctx.sync()
         return ctx.elapsed_ms(6, 7)
 
-    best, cap = [0, 0, 0, 0], 0
+    best, cap = [0, 0, 0, 0, 0], 0
     times: dict[str, list[float]] = {}
-    for cls, count in enumerate(_counts()):
+    for cls, opts in enumerate(_options()):
         t_cls = []
-        for v in range(count):
+        for v in opts:
             cfg = list(best)
             cfg[cls] = v
             t_cls.append(timed(cfg, cap))
-        times[("face_i", "face_j", "face_k", "node")[cls]] = t_cls
-        best[cls] = min(range(count), key=lambda v: t_cls[v])
+        times[NAMES[cls]] = t_cls
+        best[cls] = opts[min(range(len(opts)), key=lambda v: t_cls[v])]
     t_cap = [timed(best, c) for c in GRID_CAPS]
     times["grid_cap"] = t_cap
     cap = GRID_CAPS[min(range(len(GRID_CAPS)), key=lambda i: t_cap[i])]

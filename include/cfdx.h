@@ -292,6 +292,9 @@ int cfdx_block_timing_split(cfdx_block *b, double ms[5]);
 typedef struct {
     int32_t face[3];   /* variant of the i, j, k face sweeps */
     int32_t node;      /* variant of the node kernels */
+    int32_t strip;     /* order of the node tiles: strips of this many tile rows (along j), each
+                          walked plane by plane, so that a strip's k +- 2 planes are recent in the
+                          L2; 0 = whole planes one after another; 0 .. 4096 */
 } cfdx_tiles;
 int cfdx_tile_variants(int kind, int32_t *count, int32_t *shapes);
 int cfdx_block_set_tiles(cfdx_block *b, const cfdx_tiles *t);

@@ -38,10 +38,15 @@ def all_configs():
     cfgs = []
     for cls, n in enumerate(counts):
         for v in range(1, n):
-            c = [0, 0, 0, 0]
+            c = [0, 0, 0, 0, 0]
             c[cls] = v
             cfgs.append(tuple(c))
-    cfgs.append((counts[0] - 1, counts[1] - 1, counts[2] - 1, counts[3] - 1))
+    # node-tile strips (rows of tiles per strip, each strip walked plane by plane), with each node
+    # variant, including a strip taller than the block
+    for v in range(counts[3]):
+        for s in (1, 3, 64):
+            cfgs.append((0, 0, 0, v, s))
+    cfgs.append((counts[0] - 1, counts[1] - 1, counts[2] - 1, counts[3] - 1, 2))
     return cfgs
 
 
@@ -78,7 +83,7 @@ def test_batch_tiles_and_grid(ctx):
         dt = b.cfl_dt(0.5)
         b.step(3, 0.2, dt, prm)
         alone.append(b.get_flow())
-    for cfg, cap in [((1, 2, 3, 1), 0), ((2, 1, 2, 3), 7), ((0, 3, 1, 2), 64)]:
+    for cfg, cap in [((1, 2, 3, 1, 0), 0), ((2, 1, 2, 3, 2), 7), ((0, 3, 1, 2, 5), 64), ((0, 0, 0, 0, 1), 0)]:
         q = cfdx.Batch(cfdx.MIXED, ctx)
         blocks = []
         for s in shapes:
@@ -103,7 +108,7 @@ def test_tuner_and_cache(ctx, tmp_path):
     dt = b.cfl_dt(0.5)
     res = tune.tune_block(b, 0.2, dt, prm, steps=1, cache=cache)
     assert not res["cached"] and b.get_tiles() == tuple(res["tiles"])
-    assert set(res["times_ms"]) == {"face_i", "face_j", "face_k", "node"}
+    assert set(res["times_ms"]) == {"face_i", "face_j", "face_k", "node", "strip"}
     assert all(t > 0 for ts in res["times_ms"].values() for t in ts)
     again = tune.TileCache(str(tmp_path / "tiles.json"))
     b2 = ragged(ctx, cfdx.MIXED, 32, 24, 16)
@@ -115,7 +120,7 @@ def test_tuner_and_cache(ctx, tmp_path):
     q.add(b)
     q.add(b3)
     rq = tune.tune_batch(q, ctx, key="test-batch", steps=1, cache=cache)
-    assert rq["grid_cap"] in tune.GRID_CAPS and len(rq["tiles"]) == 4
+    assert rq["grid_cap"] in tune.GRID_CAPS and len(rq["tiles"]) == 5 and rq["tiles"][4] in tune.STRIPS
     q.close()
 
 
