@@ -190,3 +190,26 @@ def test_cgrid_trailing_edge_follows_the_tip_rule(ctx):
             lo, hi = f[at(i, 0, k)], f[at(ni - 1 - i, 0, k)]
             for c in FIELDS:
                 assert lo[c] == hi[c], (i, k, c)
+
+
+@pytest.mark.parametrize("tiles", [(0, 0, 0, 1, 3), (2, 3, 1, 0, 2), (1, 2, 2, 3, 64)])
+def test_cgrid_tiles_and_strips_same_bits(ctx, tiles):
+    """The C-grid block (bounded, wake cut, shell tiles) gives bitwise the state of the default tiles
+    with other tile variants and node-tile strips, including the plane-staged viscous tiles."""
+    d, xyz, geo, mach, aoa, kw = cgrid_case()
+    prm = cfdx.default_params(**kw)
+    flow = perturbed(xyz[cfdx.owned_index(d)], mach, aoa)
+    ref, dt = run_block(ctx, d, geo, flow, mach, None, prm, 3)
+    want = ref.get_flow()
+    ref.close()
+    dd = cfdx.BlockDesc.from_buffer_copy(d)
+    b = cfdx.Block(d.ni, d.nj, d.nk, ctx=ctx, desc=dd)
+    b.set_geometry_soa(geo)
+    b.configure(mach, 0.0, prm)
+    b.set_flow(flow)
+    b.set_tiles(tiles)
+    b.step(3, mach, dt, prm)
+    got = b.get_flow()
+    b.close()
+    for f in FIELDS:
+        assert np.array_equal(got[f], want[f]), (tiles, f)
