@@ -183,7 +183,6 @@ def cpu_baseline(cfdx, which=("c1", "c2", "ns256"), single_thread=("c1", "c2", "
     constants written into them (Re 500 and Mach 1.5 in the viscous terms), so their state is not
     compared, only their speed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import pyoracle
     from cfdx import workloads
     if not pyoracle.available("port"):
@@ -441,7 +440,6 @@ def batched_c4(cfdx, steps: int, warmup: int, nblocks: int = 64, grid_cap: int =
     Bernoulli(1/2), Mach U(0.1,0.6), Re U(500,5000)), set up on the device; mixed precision, one
     batched launch per kernel class replayed from a hipGraph of one RK3 step, tuned tiles. Also
     times the same solves in plain FP64 one block after another."""
-    import numpy as np
     from cfdx import tune, workloads
     ctx = cfdx.Ctx(0)
     specs = workloads.c4_specs(workloads.SEED, nblocks)
