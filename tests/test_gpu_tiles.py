@@ -264,3 +264,14 @@ def test_destroyed_block_leaves_its_batches(ctx):
         assert np.array_equal(got[f], want[f]), f
     a.close()
     alone.close()
+
+
+def test_strip_out_of_range_refused(ctx):
+    b = ragged(ctx, cfdx.FP64)
+    with pytest.raises(cfdx.CfdxError) as e:
+        b.set_tiles((0, 0, 0, 0, 5000))
+    assert e.value.code == cfdx.E_INVALID and "strip" in str(e.value)
+    with pytest.raises(cfdx.CfdxError):
+        b.set_tiles((0, 0, 0, 0, -1))
+    assert b.get_tiles() == (0, 0, 0, 0, 0)
+    b.close()
