@@ -17,10 +17,12 @@ per-field errors of mixed vs FP64 ("mixed_vs_fp64"), the 128^3 case in FP64 and 
 the 128^3 case through the drop-in kernels ("dropin_path"), the 64-cube batched mixed graph step
 ("batched_c4"), the NACA 0012 C-grid in mixed and FP64 with per-field errors ("airfoil_c3"), and
 the 200-request Poisson stream of NACA C-grid solves served by continuous batching vs plain FP64
-one at a time ("serving": p50 / p99 latency, throughput). cpu_baseline: the reference's kernels
-on the host cores for the 64^3 vortex (100 steps), the 128^3 case (20) and 256^3 (2), all cores
-and then one thread, each beside the GPU block path on the same case, dt and step count (rates
-side by side and the per-field differences of the two final states).
+one at a time ("serving": p50 / p99 latency, throughput). cpu_baseline: the C restatement of the
+reference's kernels and stage sequence (pinned bitwise to the reference's own kernels) on the host
+cores for the 64^3 vortex (100 steps), the 128^3 case (20) and 256^3 (2), all cores and then one
+thread, each beside the GPU block path on the same case, dt and step count (rates side by side and
+the per-field differences of the two final states); the reference's own kernels are timed on the
+128^3 case too.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ns256|c2|c1|c3] [--precision fp64|mixed]
                     [--legs compare,c2,dropin,c4,c3,serve|none] [--cpu c1,c2,ns256|none] [--no-tune]
