@@ -368,14 +368,15 @@ __device__ __forceinline__ uint32_t tile_node(const BlockDev &B, int along, int 
 }
 
 template <typename S, int DIR, int TA, int TB>
-__global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+__global__ void __launch_bounds__(256, 3) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                               uint32_t nblk, uint32_t total)
 {
     static_assert(TA * TB == 256, "face tiles have 256 threads");
     constexpr int NP = TB + 5;
     // mixed form: the Roe state's inputs (rho, u, v, w, h) of the stencil points are staged too, so
-    // phase 2 reads the two centre nodes from LDS (in FP64 the extra live range costs a wave per
-    // SIMD, so it loads them from global memory instead)
+    // phase 2 reads the two centre nodes from LDS; FP64 derives them from the conserved states of
+    // the two centre nodes, which phase 2 holds in registers anyway (no primitive loads; staging
+    // them in FP64 would cost a wave per SIMD). At most 168 VGPRs: 3 waves per SIMD in FP64.
     constexpr bool STAGE_ROE = std::is_same<S, float>::value;
     __shared__ S sq[5 * NP * TA];
     __shared__ S sf[5 * NP * TA];
@@ -442,10 +443,11 @@ __global__ void __launch_bounds__(256) k_face(const BlockDev *__restrict__ blks,
                 auto at = [&](int c, int m) { return sp[lds_at<DIR, TA, TB>(c, b + m, a)]; };
                 roe_average(at(0, 2), at(1, 2), at(2, 2), at(3, 2), at(4, 2), at(0, 3), at(1, 3), at(2, 3), at(3, 3), at(4, 3), roe);
             } else {
-                const uint32_t gr = tile_node<DIR>(B, along_shift(B, DIR, b0 + b - sh, 1, n_along), a0 + a, fixed);
-                const S ra = ld(P0, gl), ua = ld(P1, gl), va = ld(P2, gl), wa = ld(P3, gl), pa = ld(P4, gl);
-                const S rb = ld(P0, gr), ub = ld(P1, gr), vb = ld(P2, gr), wb = ld(P3, gr), pb = ld(P4, gr);
-                roe_average(ra, ua, va, wa, enthalpy(ra, ua, va, wa, pa), rb, ub, vb, wb, enthalpy(rb, ub, vb, wb, pb), roe);
+                // from the conserved states of the two centre nodes, already in registers
+                S ra, ua, va, wa, ha, rb, ub, vb, wb, hb;
+                q_roe_inputs(q[2], ra, ua, va, wa, ha);
+                q_roe_inputs(q[3], rb, ub, vb, wb, hb);
+                roe_average(ra, ua, va, wa, ha, rb, ub, vb, wb, hb, roe);
             }
             // face metric row and face det; streaming accesses on periodic blocks (B.code null)
             S fg[4];

@@ -86,10 +86,12 @@ __global__ void k_opcount(Ops *out, ViscParams vp)
             q[m][0] = in(1.0 * s); q[m][1] = in(0.9 * s); q[m][2] = in(0.1 * s); q[m][3] = in(0.05 * s); q[m][4] = in(3.2 * s);
             for (int c = 0; c < 5; ++c) nf[m][c] = in(0.3 * (c + 1) * s);
         }
-        const Cnt ha = enthalpy(in(1.0), in(0.9), in(0.1), in(0.05), in(1.1));
-        const Cnt hb = enthalpy(in(1.01), in(0.91), in(0.1), in(0.05), in(1.12));
+        // the FP64 sweep takes the Roe inputs of the two centre nodes from their conserved states
+        Cnt ra, ua, va, wa, ha, rb, ub, vb, wb, hb;
+        q_roe_inputs(q[2], ra, ua, va, wa, ha);
+        q_roe_inputs(q[3], rb, ub, vb, wb, hb);
         Cnt roe[5];
-        roe_average(in(1.0), in(0.9), in(0.1), in(0.05), ha, in(1.01), in(0.91), in(0.1), in(0.05), hb, roe);
+        roe_average(ra, ua, va, wa, ha, rb, ub, vb, wb, hb, roe);
         Cnt llf[5];
         face_llf(q, roe, in(0.9), in(0.1), in(0.05), in(1.2), Cnt(1e-7), llf);
         Cnt fh[5];

@@ -91,6 +91,19 @@ __device__ __forceinline__ T enthalpy(T rho, T u, T v, T w, T p)
     return (p * T(3.5) + T(0.5) * rho * (u * u + v * v + w * w)) * pinv_(rho);   // (p/0.4 + ke + p) / rho
 }
 
+// Roe-average inputs (rho, u, v, w, H) of a node from its conserved state Q: the face kernels hold
+// the Q of the stencil points already, so the two centre nodes need no primitive loads.
+template <typename T>
+__device__ __forceinline__ void q_roe_inputs(const T (&q)[5], T &rho, T &u, T &v, T &w, T &h)
+{
+    const T r = pinv_(q[0]);
+    rho = q[0];
+    u = q[1] * r;
+    v = q[2] * r;
+    w = q[3] * r;
+    h = (T(1.4) * q[4] - T(0.2) * (q[1] * u + q[2] * v + q[3] * w)) * r;   // (E + p) / rho, p = 0.4 (E - rho |u|^2 / 2)
+}
+
 // Roe average of the face between states a and b (U_Q.cl:60-109): (u, v, w, h, rho).
 template <typename T>
 __device__ __forceinline__ void roe_average(T ra, T ua, T va, T wa, T ha, T rb, T ub, T vb, T wb, T hb, T (&r)[5])
