@@ -138,49 +138,43 @@ __device__ __forceinline__ void node_flux(T rho, T u, T v, T w, T p, T mx, T my,
 template <typename T>
 __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg)
 {
-    const T c3o8 = T(3.0 / 8.0), c3o4 = T(3.0 / 4.0), c1o8 = T(1.0 / 8.0), c15o8 = T(15.0 / 8.0), c5o4 = T(5.0 / 4.0);
-    const T h_r = c15o8 * W[3] - c5o4 * W[4] + c3o8 * W[5];   // pos q0 (i+1, i+2, i+3)
-    const T h_c = c3o8 * W[2] + c3o4 * W[3] - c1o8 * W[4];    // pos q1 = neg q0 (i, i+1, i+2)
-    const T h_l = -c1o8 * W[1] + c3o4 * W[2] + c3o8 * W[3];   // pos q2 = neg q1 (i-1, i, i+1)
-    const T h_ll = c3o8 * W[0] - c5o4 * W[1] + c15o8 * W[2];  // neg q2 (i-2, i-1, i)
-    // IS = 13/12 s^2 + 1/4 t^2 is formed as 13/12 s^2 + (t/2)^2 with t/2 taken by halving the
-    // coefficients (exact scalings by 1/2: the same bits, one multiply fewer per indicator)
+    // the candidates, the second differences and the indicators' first-derivative terms are the
+    // same linear forms of W, written in the first differences d_m = W[m+1] - W[m]
+    const T d0 = W[1] - W[0], d1 = W[2] - W[1], d2 = W[3] - W[2], d3 = W[4] - W[3], d4 = W[5] - W[4];
+    const T h_r = W[3] + T(3.0 / 8.0) * d4 - T(7.0 / 8.0) * d3;    // pos q0 = (15 W3 - 10 W4 + 3 W5) / 8
+    const T h_c = W[3] - T(3.0 / 8.0) * d2 - T(1.0 / 8.0) * d3;    // pos q1 = neg q0 = (3 W2 + 6 W3 - W4) / 8
+    const T h_l = W[2] + T(1.0 / 8.0) * d1 + T(3.0 / 8.0) * d2;    // pos q2 = neg q1 = (-W1 + 6 W2 + 3 W3) / 8
+    const T h_ll = W[2] + T(7.0 / 8.0) * d1 - T(3.0 / 8.0) * d0;   // neg q2 = (3 W0 - 10 W1 + 15 W2) / 8
+    // IS = 13/12 s^2 + 1/4 t^2: 13/12 s_k^2 of the second differences s_k = d_{k+1} - d_k, each shared
+    // by two indicators, plus (t / 2)^2
     const T c13 = T(13.0 / 12.0);
-    const T s0 = W[0] - T(2) * W[1] + W[2];
-    const T s1 = W[1] - T(2) * W[2] + W[3];
-    const T s2 = W[2] - T(2) * W[3] + W[4];
-    const T s3 = W[3] - T(2) * W[4] + W[5];
-    // pos indicators
-    const T t0 = T(1.5) * W[3] - T(2) * W[4] + T(0.5) * W[5];
-    const T t1 = T(0.5) * (W[2] - W[4]);
-    const T t2 = T(0.5) * W[1] - T(2) * W[2] + T(1.5) * W[3];
-    const T ISp0 = c13 * s3 * s3 + t0 * t0;
-    const T ISp1 = c13 * s2 * s2 + t1 * t1;
-    const T ISp2 = c13 * s1 * s1 + t2 * t2;
-    // neg indicators
-    const T u0 = T(1.5) * W[2] - T(2) * W[3] + T(0.5) * W[4];
-    const T u1 = T(0.5) * (W[1] - W[3]);
-    const T u2 = T(0.5) * W[0] - T(2) * W[1] + T(1.5) * W[2];
-    const T ISm0 = c13 * s2 * s2 + u0 * u0;
-    const T ISm1 = c13 * s1 * s1 + u1 * u1;
-    const T ISm2 = c13 * s0 * s0 + u2 * u2;
+    const T s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2, s3 = d4 - d3;
+    const T S0 = c13 * s0 * s0, S1 = c13 * s1 * s1, S2 = c13 * s2 * s2, S3 = c13 * s3 * s3;
+    const T t0 = T(0.5) * d4 - T(1.5) * d3;   // (3 W3 - 4 W4 + W5) / 2
+    const T t1 = T(0.5) * (d2 + d3);          // (W4 - W2) / 2
+    const T t2 = T(1.5) * d2 - T(0.5) * d1;   // (W1 - 4 W2 + 3 W3) / 2
+    const T u0 = T(0.5) * d3 - T(1.5) * d2;   // (3 W2 - 4 W3 + W4) / 2
+    const T u1 = T(0.5) * (d1 + d2);          // (W3 - W1) / 2
+    const T u2 = T(1.5) * d1 - T(0.5) * d0;   // (W0 - 4 W1 + 3 W2) / 2
+    const T ISp0 = S3 + t0 * t0, ISp1 = S2 + t1 * t1, ISp2 = S1 + t2 * t2;
+    const T ISm0 = S2 + u0 * u0, ISm1 = S1 + u1 * u1, ISm2 = S0 + u2 * u2;
     // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2, so each side needs one division; the
-    // weights of each side are taken 16 times larger (d = 1, 10, 5): the factor cancels exactly
-    // in the quotients
+    // linear weights are taken 16 times larger (pos 1, 10, 5; neg 5, 10, 1): the factor cancels in
+    // the quotients
     T np_, dp_, nn_, dn_;
     {
         T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
         a0 *= a0; a1 *= a1; a2 *= a2;
-        const T w0 = a1 * a2, w1 = T(10) * (a0 * a2), w2 = T(5) * (a0 * a1);
-        np_ = w0 * h_r + w1 * h_c + w2 * h_l;
-        dp_ = w0 + w1 + w2;
+        const T e = a1 * a2, A2 = T(10) * a2, A1 = T(5) * a1;
+        np_ = e * h_r + a0 * (A2 * h_c + A1 * h_l);
+        dp_ = e + a0 * (A2 + A1);
     }
     {
-        T a0 = eps + ISm0, a1 = eps + ISm1, a2 = eps + ISm2;
-        a0 *= a0; a1 *= a1; a2 *= a2;
-        const T w0 = T(5) * (a1 * a2), w1 = T(10) * (a0 * a2), w2 = a0 * a1;
-        nn_ = w0 * h_c + w1 * h_l + w2 * h_ll;
-        dn_ = w0 + w1 + w2;
+        T b0 = eps + ISm0, b1 = eps + ISm1, b2 = eps + ISm2;
+        b0 *= b0; b1 *= b1; b2 *= b2;
+        const T e = b0 * b1, B1 = T(5) * b1, B0 = T(10) * b0;
+        nn_ = e * h_ll + b2 * (B1 * h_c + B0 * h_l);
+        dn_ = e + b2 * (B1 + B0);
     }
     if constexpr (!std::is_same<T, float>::value) {
         // one reciprocal serves both sides
