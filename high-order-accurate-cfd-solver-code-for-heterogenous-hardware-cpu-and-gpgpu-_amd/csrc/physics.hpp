@@ -217,39 +217,48 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
     const T phi_g = phi * T(2.5);
     const T hh = (phi + a * a) * T(2.5);
 
+    // Projection W = L q of the 6 stencil states, written through L's structure: rows 0-2 are
+    // k_c E + (1/rho) (k x q_mom - (k x u) q0)_c with E = (1 - phi/a^2) q0 + (0.4/a^2) m and
+    // m = u.q_mom - q4; rows 3 and 4 are beta (phi q0 - 0.4 m) -/+ beta a (theta q0 - k.q_mom).
+    const T ikx = irho * kx, iky = irho * ky, ikz = irho * kz;
+    const T icu = irho * cu, icv = irho * cv, icw = irho * cw;
+    const T bphi = beta * phi, b04 = beta * T(0.4), ba = beta * a;
+    T W[5][6];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) { qp[m] = T(0); qn[m] = T(0); }
-
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        T l0, l1, l2, l3, l4;   // row c of L
-        T r0, r1, r2, r3, r4;   // column c of R
-        if (c == 0) {
-            l0 = kx * one_m - cv * irho; l1 = kx * g4aa * u; l2 = kx * g4aa * v + kz * irho; l3 = kx * g4aa * w - ky * irho; l4 = -kx * g4aa;
-            r0 = kx; r1 = kx * u; r2 = kx * v + kz * rho; r3 = kx * w - ky * rho; r4 = kx * phi_g + rho * cv;
-        } else if (c == 1) {
-            l0 = ky * one_m - cw * irho; l1 = ky * g4aa * u - kz * irho; l2 = ky * g4aa * v; l3 = ky * g4aa * w + kx * irho; l4 = -ky * g4aa;
-            r0 = ky; r1 = ky * u - kz * rho; r2 = ky * v; r3 = ky * w + kx * rho; r4 = ky * phi_g + rho * cw;
-        } else if (c == 2) {
-            l0 = kz * one_m - cu * irho; l1 = kz * g4aa * u + ky * irho; l2 = kz * g4aa * v - kx * irho; l3 = kz * g4aa * w; l4 = -kz * g4aa;
-            r0 = kz; r1 = kz * u + ky * rho; r2 = kz * v - kx * rho; r3 = kz * w; r4 = kz * phi_g + rho * cu;
-        } else if (c == 3) {
-            l0 = beta * (phi - theta * a); l1 = -beta * (T(0.4) * u - kx * a); l2 = -beta * (T(0.4) * v - ky * a);
-            l3 = -beta * (T(0.4) * w - kz * a); l4 = beta * T(0.4);
-            r0 = alpha; r1 = alpha * (u + kx * a); r2 = alpha * (v + ky * a); r3 = alpha * (w + kz * a); r4 = alpha * (hh + theta * a);
-        } else {
-            l0 = beta * (phi + theta * a); l1 = -beta * (T(0.4) * u + kx * a); l2 = -beta * (T(0.4) * v + ky * a);
-            l3 = -beta * (T(0.4) * w + kz * a); l4 = beta * T(0.4);
-            r0 = alpha; r1 = alpha * (u - kx * a); r2 = alpha * (v - ky * a); r3 = alpha * (w - kz * a); r4 = alpha * (hh - theta * a);
-        }
-        T W[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) W[m] = l0 * q[m][0] + l1 * q[m][1] + l2 * q[m][2] + l3 * q[m][3] + l4 * q[m][4];
-        T cp, cn;
-        wcns_pair(W, eps, cp, cn);
-        qp[0] += r0 * cp; qp[1] += r1 * cp; qp[2] += r2 * cp; qp[3] += r3 * cp; qp[4] += r4 * cp;
-        qn[0] += r0 * cn; qn[1] += r1 * cn; qn[2] += r2 * cn; qn[3] += r3 * cn; qn[4] += r4 * cn;
+    for (int m = 0; m < 6; ++m) {
+        const T q0 = q[m][0], qx = q[m][1], qy = q[m][2], qz = q[m][3], q4 = q[m][4];
+        const T mm = u * qx - q4 + v * qy + w * qz;
+        const T nn = kx * qx + ky * qy + kz * qz;
+        const T E = one_m * q0 + g4aa * mm;
+        W[0][m] = kx * E + (ikz * qy - iky * qz - icv * q0);
+        W[1][m] = ky * E + (ikx * qz - ikz * qx - icw * q0);
+        W[2][m] = kz * E + (iky * qx - ikx * qy - icu * q0);
+        const T bA = bphi * q0 - b04 * mm;
+        const T Bv = theta * q0 - nn;
+        W[3][m] = bA - ba * Bv;
+        W[4][m] = bA + ba * Bv;
     }
+    T xp[5], xn[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) wcns_pair(W[c], eps, xp[c], xn[c]);
+    // Back projection Q = R x through R's structure: with X = k.(x0, x1, x2),
+    // P = alpha (x3 + x4) and M = alpha a (x3 - x4): Q0 = X + P, Q_mom = u Q0 + k M + rho (k x x),
+    // Q4 = (phi/0.4) X + H P + theta M + rho (k x u).(x0, x1, x2)
+    const T rkx = rho * kx, rky = rho * ky, rkz = rho * kz;
+    const T rcu = rho * cu, rcv = rho * cv, rcw = rho * cw;
+    const T aa = alpha * a;
+    auto back = [&](const T (&x)[5], T (&Q)[5]) {
+        const T X = kx * x[0] + ky * x[1] + kz * x[2];
+        const T P = alpha * (x[3] + x[4]);
+        const T M = aa * (x[3] - x[4]);
+        Q[0] = X + P;
+        Q[1] = u * Q[0] + kx * M + rky * x[2] - rkz * x[1];
+        Q[2] = v * Q[0] + ky * M + rkz * x[0] - rkx * x[2];
+        Q[3] = w * Q[0] + kz * M + rkx * x[1] - rky * x[0];
+        Q[4] = phi_g * X + hh * P + theta * M + rcv * x[0] + rcw * x[1] + rcu * x[2];
+    };
+    back(xp, qp);
+    back(xn, qn);
 }
 
 // det_f * (k . (F, E, G))(Q) and lambda_5 = k.u + c |k| of one interface state
