@@ -413,8 +413,8 @@ __global__ void __launch_bounds__(256, 3) k_face(const BlockDev *__restrict__ bl
             const S rho = ld(P0, x), u = ld(P1, x), v = ld(P2, x), w = ld(P3, x), pr = ld(P4, x);
             S q[5], f[5];
             prim_to_q(rho, u, v, w, pr, q);
-            node_flux(rho, u, v, w, pr, ld(geo<S>(B, 1 + 3 * DIR), x), ld(geo<S>(B, 2 + 3 * DIR), x),
-                      ld(geo<S>(B, 3 + 3 * DIR), x), ld(geo<S>(B, 0), x), f);
+            node_flux_e(rho, u, v, w, pr, q[4] + pr, ld(geo<S>(B, 1 + 3 * DIR), x), ld(geo<S>(B, 2 + 3 * DIR), x),
+                        ld(geo<S>(B, 3 + 3 * DIR), x), ld(geo<S>(B, 0), x), f);
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
                 sq[lds_at<DIR, TA, TB>(c, s, a)] = q[c];

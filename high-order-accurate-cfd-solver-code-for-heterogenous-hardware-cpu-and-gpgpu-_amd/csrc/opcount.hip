@@ -103,8 +103,9 @@ __global__ void k_opcount(Ops *out, ViscParams vp)
         Cnt q[5];
         prim_to_q(in(1.0), in(0.9), in(0.1), in(0.05), in(1.1), q);
         take(out[1]);
+        // the sweeps form E + p from the conserved energy just computed
         Cnt f[5];
-        node_flux(in(1.0), in(0.9), in(0.1), in(0.05), in(1.1), in(0.9), in(0.1), in(0.05), in(1.2), f);
+        node_flux_e(in(1.0), in(0.9), in(0.1), in(0.05), in(1.1), q[4] + in(1.1), in(0.9), in(0.1), in(0.05), in(1.2), f);
         take(out[2]);
     }
     {

@@ -116,7 +116,18 @@ __device__ __forceinline__ void roe_average(T ra, T ua, T va, T wa, T ha, T rb, 
 }
 
 // Contravariant, det-scaled inviscid node flux along metric row m (Viscous.cl:560-611):
-// D * (mx F1 + my E1 + mz G1) with F1, E1, G1 the Cartesian Euler fluxes.
+// D * (mx F1 + my E1 + mz G1) with F1, E1, G1 the Cartesian Euler fluxes; Ep = E + p.
+template <typename T>
+__device__ __forceinline__ void node_flux_e(T rho, T u, T v, T w, T p, T Ep, T mx, T my, T mz, T D, T (&f)[5])
+{
+    const T V = mx * u + my * v + mz * w;
+    const T rV = rho * V;
+    f[0] = D * rV;
+    f[1] = D * (rV * u + mx * p);
+    f[2] = D * (rV * v + my * p);
+    f[3] = D * (rV * w + mz * p);
+    f[4] = D * (Ep * V);
+}
 template <typename T>
 __device__ __forceinline__ void node_flux(T rho, T u, T v, T w, T p, T mx, T my, T mz, T D, T (&f)[5])
 {
@@ -145,32 +156,35 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     const T h_c = W[3] - T(3.0 / 8.0) * d2 - T(1.0 / 8.0) * d3;    // pos q1 = neg q0 = (3 W2 + 6 W3 - W4) / 8
     const T h_l = W[2] + T(1.0 / 8.0) * d1 + T(3.0 / 8.0) * d2;    // pos q2 = neg q1 = (-W1 + 6 W2 + 3 W3) / 8
     const T h_ll = W[2] + T(7.0 / 8.0) * d1 - T(3.0 / 8.0) * d0;   // neg q2 = (3 W0 - 10 W1 + 15 W2) / 8
-    // IS = 13/12 s^2 + 1/4 t^2: 13/12 s_k^2 of the second differences s_k = d_{k+1} - d_k, each shared
-    // by two indicators, plus (t / 2)^2
-    const T c13 = T(13.0 / 12.0);
+    // the indicators are formed four times larger, 4 IS = 13/3 s^2 + t^2, from the second
+    // differences s_k = d_{k+1} - d_k (each 13/3 s_k^2 shared by two indicators) and the
+    // first-derivative terms t; with eps also taken four times larger every (eps + IS)^2 carries the
+    // same factor 16, which cancels in the weights
+    const T c13 = T(13.0 / 3.0);
     const T s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2, s3 = d4 - d3;
     const T S0 = c13 * s0 * s0, S1 = c13 * s1 * s1, S2 = c13 * s2 * s2, S3 = c13 * s3 * s3;
-    const T t0 = T(0.5) * d4 - T(1.5) * d3;   // (3 W3 - 4 W4 + W5) / 2
-    const T t1 = T(0.5) * (d2 + d3);          // (W4 - W2) / 2
-    const T t2 = T(1.5) * d2 - T(0.5) * d1;   // (W1 - 4 W2 + 3 W3) / 2
-    const T u0 = T(0.5) * d3 - T(1.5) * d2;   // (3 W2 - 4 W3 + W4) / 2
-    const T u1 = T(0.5) * (d1 + d2);          // (W3 - W1) / 2
-    const T u2 = T(1.5) * d1 - T(0.5) * d0;   // (W0 - 4 W1 + 3 W2) / 2
+    const T t0 = d4 - T(3) * d3;   // 3 W3 - 4 W4 + W5
+    const T t1 = d2 + d3;          // W4 - W2
+    const T t2 = T(3) * d2 - d1;   // W1 - 4 W2 + 3 W3
+    const T u0 = d3 - T(3) * d2;   // 3 W2 - 4 W3 + W4
+    const T u1 = d1 + d2;          // W3 - W1
+    const T u2 = T(3) * d1 - d0;   // W0 - 4 W1 + 3 W2
     const T ISp0 = S3 + t0 * t0, ISp1 = S2 + t1 * t1, ISp2 = S1 + t2 * t2;
     const T ISm0 = S2 + u0 * u0, ISm1 = S1 + u1 * u1, ISm2 = S0 + u2 * u2;
+    const T eps4 = T(4) * eps;
     // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2, so each side needs one division; the
     // linear weights are taken 16 times larger (pos 1, 10, 5; neg 5, 10, 1): the factor cancels in
     // the quotients
     T np_, dp_, nn_, dn_;
     {
-        T a0 = eps + ISp0, a1 = eps + ISp1, a2 = eps + ISp2;
+        T a0 = eps4 + ISp0, a1 = eps4 + ISp1, a2 = eps4 + ISp2;
         a0 *= a0; a1 *= a1; a2 *= a2;
         const T e = a1 * a2, A2 = T(10) * a2, A1 = T(5) * a1;
         np_ = e * h_r + a0 * (A2 * h_c + A1 * h_l);
         dp_ = e + a0 * (A2 + A1);
     }
     {
-        T b0 = eps + ISm0, b1 = eps + ISm1, b2 = eps + ISm2;
+        T b0 = eps4 + ISm0, b1 = eps4 + ISm1, b2 = eps4 + ISm2;
         b0 *= b0; b1 *= b1; b2 *= b2;
         const T e = b0 * b1, B1 = T(5) * b1, B0 = T(10) * b0;
         nn_ = e * h_ll + b2 * (B1 * h_c + B0 * h_l);
@@ -261,24 +275,25 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
     back(xn, qn);
 }
 
-// det_f * (k . (F, E, G))(Q) and lambda_5 = k.u + c |k| of one interface state
+// k . (F, E, G)(Q) (not det-scaled) and lambda_5 = k.u + c |k| of one interface state
 // (Weno.cl:2291-2507 fluxes, :2511-2581 eigenvalues, :2585-2655 contravariant form).
 template <typename T>
-__device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k2, T kk2, T detf, T (&c)[5], T &lam)
+__device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k2, T kk2, T (&c)[5], T &lam)
 {
     const T r = pinv_(Q[0]);
     const T ke = (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * r;
     const T P = T(0.4) * (Q[4] - T(0.5) * ke);
     const T Un = (k0 * Q[1] + k1 * Q[2] + k2 * Q[3]) * r;
-    c[0] = detf * (Q[0] * Un);
-    c[1] = detf * (Q[1] * Un + k0 * P);
-    c[2] = detf * (Q[2] * Un + k1 * P);
-    c[3] = detf * (Q[3] * Un + k2 * P);
-    c[4] = detf * ((T(1.4) * Q[4] - T(0.2) * ke) * Un);
+    c[0] = Q[0] * Un;
+    c[1] = Q[1] * Un + k0 * P;
+    c[2] = Q[2] * Un + k1 * P;
+    c[3] = Q[3] * Un + k2 * P;
+    c[4] = (T(1.4) * Q[4] - T(0.2) * ke) * Un;
     lam = Un + psqrt_(T(1.4) * P * r * kk2);
 }
 
-// LLF face flux with alpha = max(|lambda_5|) of the two interface states (Weno.cl:2859-2947).
+// LLF face flux with alpha = max(|lambda_5|) of the two interface states (Weno.cl:2859-2947); the
+// face det scales the sum once.
 template <typename T>
 __device__ __forceinline__ void face_llf(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
                                          T (&llf)[5])
@@ -287,12 +302,12 @@ __device__ __forceinline__ void face_llf(const T (&q)[6][5], const T (&roe)[5], 
     characteristic_states(q, roe, k0, k1, k2, eps, qp, qn);
     const T kk2 = k0 * k0 + k1 * k1 + k2 * k2;
     T cp[5], cn[5], lp, ln;
-    face_state_flux(qp, k0, k1, k2, kk2, detf, cp, lp);
-    face_state_flux(qn, k0, k1, k2, kk2, detf, cn, ln);
+    face_state_flux(qp, k0, k1, k2, kk2, cp, lp);
+    face_state_flux(qn, k0, k1, k2, kk2, cn, ln);
     const T al = fabs_(lp) >= fabs_(ln) ? fabs_(lp) : fabs_(ln);
-    const T ad = al * detf;
+    const T hd = T(0.5) * detf;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) llf[c] = T(0.5) * (cp[c] + cn[c] - ad * (qp[c] - qn[c]));
+    for (int c = 0; c < 5; ++c) llf[c] = hd * (cp[c] + cn[c] - al * (qp[c] - qn[c]));
 }
 
 // Explicit midpoint correction - delta2 F / 24 + 7 delta4 F / 5760 of one component from the node
