@@ -532,8 +532,8 @@ extern "C" int cfdx_block_set_geometry(cfdx_block *b, const double *det, const c
         for (int d = 0; d < 3; ++d) h[(19 + d) * n + g] = dd[2 * d];
     }
     CFDX_HIP(hipMemcpyAsync((void *)b->B.det, h.data(), h.size() * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry copy");
-    cfdx::blk::geometry_f32(b->B, b->ctx->stream);
-    CFDX_TRY(cfdx::check_launch("k_geo32"));
+    cfdx::blk::geometry_finalize(b->B, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("geometry_finalize"));
     CFDX_HIP(hipStreamSynchronize(b->ctx->stream), "set_geometry sync");
     return CFDX_OK;
 }
@@ -543,8 +543,8 @@ extern "C" int cfdx_block_set_geometry_soa(cfdx_block *b, const double *geo)
     if (!b || !geo) return cfdx::fail(CFDX_E_INVALID, "set_geometry_soa: null argument");
     CFDX_TRY(enter_block(b));
     CFDX_HIP(hipMemcpyAsync((void *)b->B.det, geo, (size_t)22 * b->B.n * 8, hipMemcpyHostToDevice, b->ctx->stream), "set_geometry_soa copy");
-    cfdx::blk::geometry_f32(b->B, b->ctx->stream);
-    CFDX_TRY(cfdx::check_launch("k_geo32"));
+    cfdx::blk::geometry_finalize(b->B, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("geometry_finalize"));
     // finiteness checked on the device; the counter lives in the face-flux scratch
     unsigned *cnt = (unsigned *)b->B.FH[0][0];
     CFDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), b->ctx->stream), "set_geometry_soa clear");
@@ -608,8 +608,8 @@ extern "C" int cfdx_block_init_case(cfdx_block *b, const cfdx_case_desc *c)
     CFDX_TRY(enter_block(b));
     cfdx::blk::init_geometry(b->B, c->length, c->amp, c->phase, b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_geometry"));
-    cfdx::blk::geometry_f32(b->B, b->ctx->stream);
-    CFDX_TRY(cfdx::check_launch("k_geo32"));
+    cfdx::blk::geometry_finalize(b->B, b->ctx->stream);
+    CFDX_TRY(cfdx::check_launch("geometry_finalize"));
     cfdx::blk::init_state(b->B, c->length, c->amp, c->phase, c->init, c->mach, c->beta, c->center[0], c->center[1], b->ctx->stream);
     CFDX_TRY(cfdx::check_launch("init_state"));
     cfdx::blk::base(b->B, b->desc.precision, b->ctx->stream, b->B.Us[0]);

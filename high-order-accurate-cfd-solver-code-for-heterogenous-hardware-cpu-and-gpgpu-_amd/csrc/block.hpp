@@ -111,7 +111,8 @@ struct Overlap {
 void stage(const LaunchList &L, int precision, int mode, hipStream_t st, bool any_visc, hipEvent_t *ev,
            const Overlap *ov = nullptr);
 // the float copies of the geometry of a mixed block (after every write of its double geometry)
-void geometry_f32(const BlockDev &B, hipStream_t st);
+// after every geometry write: face arrays to unit normal and det_f |k|, then the float copy
+void geometry_finalize(const BlockDev &B, hipStream_t st);
 // P and U0 of the owned nodes from double primitives staged component-major in owned order
 void base(const BlockDev &B, int precision, hipStream_t st, const double *stage);
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out);

@@ -449,7 +449,8 @@ __global__ void __launch_bounds__(256, 3) k_face(const BlockDev *__restrict__ bl
                 q_roe_inputs(q[3], rb, ub, vb, wb, hb);
                 roe_average(ra, ua, va, wa, ha, rb, ub, vb, wb, hb, roe);
             }
-            // face metric row and face det; streaming accesses on periodic blocks (B.code null)
+            // unit face normal and det_f |k| (the stored face arrays after geometry_finalize);
+            // streaming accesses on periodic blocks (B.code null)
             S fg[4];
             if (B.code) {
 #pragma unroll
@@ -1019,12 +1020,36 @@ __global__ void __launch_bounds__(256) k_geo32(const double *__restrict__ src, f
         dst[q] = (float)src[q];
 }
 
-void geometry_f32(const BlockDev &B, hipStream_t st)
+// The face arrays (metric row k of the +1/2 face along each direction, arrays 10-18, and its det
+// det_f, arrays 19-21) into the form the face sweeps use: the unit normal k / |k| and det_f |k|.
+// A stored node with no face (k = 0) keeps zeros.
+__global__ void __launch_bounds__(256) k_face_unit(double *__restrict__ geo, size_t n)
 {
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double *k = geo + (size_t)(10 + 3 * d) * n + g, *df = geo + (size_t)(19 + d) * n + g;
+            const double a = k[0], b = k[n], c = k[2 * n];
+            const double kn = sqrt(a * a + b * b + c * c);
+            if (kn > 0.0) {
+                k[0] = a / kn;
+                k[n] = b / kn;
+                k[2 * n] = c / kn;
+                df[0] = df[0] * kn;
+            }
+        }
+    }
+}
+
+void geometry_finalize(const BlockDev &B, hipStream_t st)
+{
+    const size_t n = B.n;
+    const unsigned nf = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_face_unit, dim3(nf), dim3(256), 0, st, const_cast<double *>(B.det), n);
     if (!B.geo32) return;
-    const size_t n = (size_t)22 * B.n;
-    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_geo32, dim3(nb), dim3(256), 0, st, B.det, const_cast<float *>(B.geo32), n);
+    const size_t n22 = (size_t)22 * B.n;
+    const unsigned nb = (unsigned)std::min<size_t>((n22 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_geo32, dim3(nb), dim3(256), 0, st, B.det, const_cast<float *>(B.geo32), n22);
 }
 
 void prims(const BlockDev &B, int precision, hipStream_t st, double *out)

@@ -93,7 +93,8 @@ __global__ void k_opcount(Ops *out, ViscParams vp)
         Cnt roe[5];
         roe_average(ra, ua, va, wa, ha, rb, ub, vb, wb, hb, roe);
         Cnt llf[5];
-        face_llf(q, roe, in(0.9), in(0.1), in(0.05), in(1.2), Cnt(1e-7), llf);
+        // the unit face normal and det_f |k| as the sweeps read them
+        face_llf(q, roe, in(0.99), in(0.11), in(0.055), in(1.2), Cnt(1e-7), llf);
         Cnt fh[5];
         for (int c = 0; c < 5; ++c)
             fh[c] = llf[c] + midpoint_correction(nf[0][c], nf[1][c], nf[2][c], nf[3][c], nf[4][c], nf[5][c]);

@@ -206,15 +206,13 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
 // Characteristic reconstruction of the two interface states of one face
 // (Weno.cl:336-455 eigenvectors and projection, :1056-2187 WCNS, :2188-2287 back projection).
 //   q[m][c]: conserved Q at offsets -2..+3 from the left node
-//   roe = (u, v, w, h, rho) of the face; k = unnormalised face metric row
+//   roe = (u, v, w, h, rho) of the face; (kx, ky, kz) = the face metric row over its length
 //   qp / qn: right-biased / left-biased conserved states (not det-scaled)
 template <typename T>
-__device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T eps,
+__device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const T (&roe)[5], T kx, T ky, T kz, T eps,
                                                       T (&qp)[5], T (&qn)[5])
 {
     const T u = roe[0], v = roe[1], w = roe[2], h = roe[3], rho = roe[4];
-    const T ikk = prsq_(k0 * k0 + k1 * k1 + k2 * k2);
-    const T kx = k0 * ikk, ky = k1 * ikk, kz = k2 * ikk;
     const T q2 = u * u + v * v + w * w;
     const T a = psqrt_(T(0.4) * (h - T(0.5) * q2));
     const T theta = kx * u + ky * v + kz * w;
@@ -275,37 +273,37 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
     back(xn, qn);
 }
 
-// k . (F, E, G)(Q) (not det-scaled) and lambda_5 = k.u + c |k| of one interface state
-// (Weno.cl:2291-2507 fluxes, :2511-2581 eigenvalues, :2585-2655 contravariant form).
+// n . (F, E, G)(Q) and lambda_5 = n.u + c of one interface state along the unit face normal n
+// (Weno.cl:2291-2507 fluxes, :2511-2581 eigenvalues, :2585-2655 contravariant form: the face's
+// |k| det_f scales the LLF sum once).
 template <typename T>
-__device__ __forceinline__ void face_state_flux(const T (&Q)[5], T k0, T k1, T k2, T kk2, T (&c)[5], T &lam)
+__device__ __forceinline__ void face_state_flux(const T (&Q)[5], T kx, T ky, T kz, T (&c)[5], T &lam)
 {
     const T r = pinv_(Q[0]);
     const T ke = (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * r;
     const T P = T(0.4) * (Q[4] - T(0.5) * ke);
-    const T Un = (k0 * Q[1] + k1 * Q[2] + k2 * Q[3]) * r;
+    const T Un = (kx * Q[1] + ky * Q[2] + kz * Q[3]) * r;
     c[0] = Q[0] * Un;
-    c[1] = Q[1] * Un + k0 * P;
-    c[2] = Q[2] * Un + k1 * P;
-    c[3] = Q[3] * Un + k2 * P;
+    c[1] = Q[1] * Un + kx * P;
+    c[2] = Q[2] * Un + ky * P;
+    c[3] = Q[3] * Un + kz * P;
     c[4] = (T(1.4) * Q[4] - T(0.2) * ke) * Un;
-    lam = Un + psqrt_(T(1.4) * P * r * kk2);
+    lam = Un + psqrt_(T(1.4) * P * r);
 }
 
-// LLF face flux with alpha = max(|lambda_5|) of the two interface states (Weno.cl:2859-2947); the
-// face det scales the sum once.
+// LLF face flux with alpha = max(|lambda_5|) of the two interface states (Weno.cl:2859-2947):
+// (kx, ky, kz) the unit face normal, dk = det_f |k| of the face metric row k.
 template <typename T>
-__device__ __forceinline__ void face_llf(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
+__device__ __forceinline__ void face_llf(const T (&q)[6][5], const T (&roe)[5], T kx, T ky, T kz, T dk, T eps,
                                          T (&llf)[5])
 {
     T qp[5], qn[5];
-    characteristic_states(q, roe, k0, k1, k2, eps, qp, qn);
-    const T kk2 = k0 * k0 + k1 * k1 + k2 * k2;
+    characteristic_states(q, roe, kx, ky, kz, eps, qp, qn);
     T cp[5], cn[5], lp, ln;
-    face_state_flux(qp, k0, k1, k2, kk2, cp, lp);
-    face_state_flux(qn, k0, k1, k2, kk2, cn, ln);
+    face_state_flux(qp, kx, ky, kz, cp, lp);
+    face_state_flux(qn, kx, ky, kz, cn, ln);
     const T al = fabs_(lp) >= fabs_(ln) ? fabs_(lp) : fabs_(ln);
-    const T hd = T(0.5) * detf;
+    const T hd = T(0.5) * dk;
 #pragma unroll
     for (int c = 0; c < 5; ++c) llf[c] = hd * (cp[c] + cn[c] - al * (qp[c] - qn[c]));
 }
@@ -321,13 +319,15 @@ __device__ __forceinline__ T midpoint_correction(T f0, T f1, T f2, T f3, T f4, T
     return T(57.0 / 11520.0) * (f0 + f5) - T(411.0 / 11520.0) * (f1 + f4) + T(354.0 / 11520.0) * (f2 + f3);
 }
 
-// Face flux F-hat = LLF + midpoint correction from node fluxes nf at offsets -2..+3.
+// Face flux F-hat = LLF + midpoint correction from node fluxes nf at offsets -2..+3; (k0, k1, k2)
+// the face metric row as stored by the reference, detf the face det.
 template <typename T, typename TO>
 __device__ __forceinline__ void face_flux(const T (&q)[6][5], const T (&roe)[5], T k0, T k1, T k2, T detf, T eps,
                                           const T (&nf)[6][5], TO (&out)[5])
 {
+    const T kn = sqrt_(k0 * k0 + k1 * k1 + k2 * k2), ik = inv_(kn);
     T llf[5];
-    face_llf(q, roe, k0, k1, k2, detf, eps, llf);
+    face_llf(q, roe, k0 * ik, k1 * ik, k2 * ik, detf * kn, eps, llf);
 #pragma unroll
     for (int c = 0; c < 5; ++c)
         out[c] = TO(llf[c] + midpoint_correction(nf[0][c], nf[1][c], nf[2][c], nf[3][c], nf[4][c], nf[5][c]));
