@@ -368,7 +368,7 @@ __device__ __forceinline__ uint32_t tile_node(const BlockDev &B, int along, int 
 }
 
 template <typename S, int DIR, int TA, int TB>
-__global__ void __launch_bounds__(256, 3) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+__global__ void __launch_bounds__(256, (TA == 64 && TB == 4) ? 3 : 4) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                               uint32_t nblk, uint32_t total)
 {
     static_assert(TA * TB == 256, "face tiles have 256 threads");
@@ -376,7 +376,9 @@ __global__ void __launch_bounds__(256, 3) k_face(const BlockDev *__restrict__ bl
     // mixed form: the Roe state's inputs (rho, u, v, w, h) of the stencil points are staged too, so
     // phase 2 reads the two centre nodes from LDS; FP64 derives them from the conserved states of
     // the two centre nodes, which phase 2 holds in registers anyway (no primitive loads; staging
-    // them in FP64 would cost a wave per SIMD). At most 168 VGPRs: 3 waves per SIMD in FP64.
+    // them in FP64 would cost a wave per SIMD). FP64 tiles fit 128 VGPRs, 4 waves per SIMD, with at
+    // most 12 bytes spilled (4-9 % faster than 3 waves at 144-166 VGPRs); the (64, 4) tile needs
+    // 130 and stays at 3 waves.
     constexpr bool STAGE_ROE = std::is_same<S, float>::value;
     __shared__ S sq[5 * NP * TA];
     __shared__ S sf[5 * NP * TA];
