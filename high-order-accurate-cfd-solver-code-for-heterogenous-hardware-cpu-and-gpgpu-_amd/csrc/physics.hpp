@@ -160,31 +160,31 @@ __device__ __forceinline__ void wcns_pair(const T (&W)[6], T eps, T &pos, T &neg
     // differences s_k = d_{k+1} - d_k (each 13/3 s_k^2 shared by two indicators) and the
     // first-derivative terms t; with eps also taken four times larger every (eps + IS)^2 carries the
     // same factor 16, which cancels in the weights
-    const T c13 = T(13.0 / 3.0);
+    const T c13 = T(13.0 / 3.0), eps4 = T(4) * eps;
     const T s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2, s3 = d4 - d3;
-    const T S0 = c13 * s0 * s0, S1 = c13 * s1 * s1, S2 = c13 * s2 * s2, S3 = c13 * s3 * s3;
+    // eps + 13/3 s_k^2, each shared by two indicators
+    const T S0 = c13 * s0 * s0 + eps4, S1 = c13 * s1 * s1 + eps4, S2 = c13 * s2 * s2 + eps4, S3 = c13 * s3 * s3 + eps4;
     const T t0 = d4 - T(3) * d3;   // 3 W3 - 4 W4 + W5
     const T t1 = d2 + d3;          // W4 - W2
     const T t2 = T(3) * d2 - d1;   // W1 - 4 W2 + 3 W3
     const T u0 = d3 - T(3) * d2;   // 3 W2 - 4 W3 + W4
     const T u1 = d1 + d2;          // W3 - W1
     const T u2 = T(3) * d1 - d0;   // W0 - 4 W1 + 3 W2
-    const T ISp0 = S3 + t0 * t0, ISp1 = S2 + t1 * t1, ISp2 = S1 + t2 * t2;
+    const T ISp0 = S3 + t0 * t0, ISp1 = S2 + t1 * t1, ISp2 = S1 + t2 * t2;   // eps + IS, four times larger
     const T ISm0 = S2 + u0 * u0, ISm1 = S1 + u1 * u1, ISm2 = S0 + u2 * u2;
-    const T eps4 = T(4) * eps;
     // w_k proportional to d_k * prod_{m != k} (eps + IS_m)^2, so each side needs one division; the
     // linear weights are taken 16 times larger (pos 1, 10, 5; neg 5, 10, 1): the factor cancels in
     // the quotients
     T np_, dp_, nn_, dn_;
     {
-        T a0 = eps4 + ISp0, a1 = eps4 + ISp1, a2 = eps4 + ISp2;
+        T a0 = ISp0, a1 = ISp1, a2 = ISp2;
         a0 *= a0; a1 *= a1; a2 *= a2;
         const T e = a1 * a2, A2 = T(10) * a2, A1 = T(5) * a1;
         np_ = e * h_r + a0 * (A2 * h_c + A1 * h_l);
         dp_ = e + a0 * (A2 + A1);
     }
     {
-        T b0 = eps4 + ISm0, b1 = eps4 + ISm1, b2 = eps4 + ISm2;
+        T b0 = ISm0, b1 = ISm1, b2 = ISm2;
         b0 *= b0; b1 *= b1; b2 *= b2;
         const T e = b0 * b1, B1 = T(5) * b1, B0 = T(10) * b0;
         nn_ = e * h_ll + b2 * (B1 * h_c + B0 * h_l);
