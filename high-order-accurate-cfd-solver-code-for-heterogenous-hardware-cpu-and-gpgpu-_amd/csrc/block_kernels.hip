@@ -367,8 +367,15 @@ __device__ __forceinline__ uint32_t tile_node(const BlockDev &B, int along, int 
     return node_index(B, across, fixed, along);
 }
 
+// Waves per SIMD the face kernel is compiled for: 4 (at most 128 VGPRs) for the tiles whose FP64
+// form fits them without spilling (every i tile, the (32, 8) tile along j and k), else 3. A
+// kernel that spills needs scratch memory, and its launches ran up to 40 % slower in a sequence of
+// sweeps than at 3 waves without.
+template <int DIR, int TA, int TB>
+constexpr int FACE_WAVES = (DIR == 0 || (TA == 32 && TB == 8)) ? 4 : 3;
+
 template <typename S, int DIR, int TA, int TB>
-__global__ void __launch_bounds__(256, (TA == 64 && TB == 4) ? 3 : 4) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
+__global__ void __launch_bounds__(256, (FACE_WAVES<DIR, TA, TB>)) k_face(const BlockDev *__restrict__ blks, const uint32_t *__restrict__ prefix,
                                               uint32_t nblk, uint32_t total)
 {
     static_assert(TA * TB == 256, "face tiles have 256 threads");
@@ -376,9 +383,7 @@ __global__ void __launch_bounds__(256, (TA == 64 && TB == 4) ? 3 : 4) k_face(con
     // mixed form: the Roe state's inputs (rho, u, v, w, h) of the stencil points are staged too, so
     // phase 2 reads the two centre nodes from LDS; FP64 derives them from the conserved states of
     // the two centre nodes, which phase 2 holds in registers anyway (no primitive loads; staging
-    // them in FP64 would cost a wave per SIMD). FP64 tiles fit 128 VGPRs, 4 waves per SIMD, with at
-    // most 12 bytes spilled (4-9 % faster than 3 waves at 144-166 VGPRs); the (64, 4) tile needs
-    // 130 and stays at 3 waves.
+    // them in FP64 would cost a wave per SIMD).
     constexpr bool STAGE_ROE = std::is_same<S, float>::value;
     __shared__ S sq[5 * NP * TA];
     __shared__ S sf[5 * NP * TA];
