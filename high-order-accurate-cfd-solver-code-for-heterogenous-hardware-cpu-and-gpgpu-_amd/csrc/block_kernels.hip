@@ -1030,7 +1030,7 @@ __global__ void __launch_bounds__(256) k_geo32(const double *__restrict__ src, f
 // The face arrays (metric row k of the +1/2 face along each direction, arrays 10-18, and its det
 // det_f, arrays 19-21) into the form the face sweeps use: the unit normal k / |k| and det_f |k|.
 // A stored node with no face (k = 0) keeps zeros.
-__global__ void __launch_bounds__(256) k_face_unit(double *__restrict__ geo, size_t n)
+__global__ void __launch_bounds__(256) k_unit_normals(double *__restrict__ geo, size_t n)
 {
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (size_t)gridDim.x * blockDim.x) {
 #pragma unroll
@@ -1052,7 +1052,7 @@ void geometry_finalize(const BlockDev &B, hipStream_t st)
 {
     const size_t n = B.n;
     const unsigned nf = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_face_unit, dim3(nf), dim3(256), 0, st, const_cast<double *>(B.det), n);
+    hipLaunchKernelGGL(k_unit_normals, dim3(nf), dim3(256), 0, st, const_cast<double *>(B.det), n);
     if (!B.geo32) return;
     const size_t n22 = (size_t)22 * B.n;
     const unsigned nb = (unsigned)std::min<size_t>((n22 + 255) / 256, 8192);
