@@ -224,7 +224,7 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
 #pragma unroll
     for (int c = 0; c < 5; ++c) U[c] = PT ? tile(c, 0, 0) : ld(Uin[c], g);
     double dU[5][3], KS[3];
-    const double iU0 = 1.0 / U[0];
+    const double iU0 = pinv_(U[0]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const uint32_t m2 = shifted(B, i, j, k, d, -2), m1 = shifted(B, i, j, k, d, -1);
@@ -260,7 +260,7 @@ __device__ __forceinline__ void visc_node(const BlockDev &B, int i, int j, int k
             }
         }
         // MTS scale-similarity term of momentum component d+1 along d (Viscous.cl:493-495)
-        const double a = (U[1 + d] * iU0) - ((0.25 * um1[1 + d] + 0.5 * U[1 + d] + 0.25 * up1[1 + d]) / (0.25 * um1[0] + 0.5 * U[0] + 0.25 * up1[0]));
+        const double a = (U[1 + d] * iU0) - ((0.25 * um1[1 + d] + 0.5 * U[1 + d] + 0.25 * up1[1 + d]) * pinv_(0.25 * um1[0] + 0.5 * U[0] + 0.25 * up1[0]));
         KS[d] = a * a;
     }
     // the differences above are taken in double; the flux arithmetic runs in S

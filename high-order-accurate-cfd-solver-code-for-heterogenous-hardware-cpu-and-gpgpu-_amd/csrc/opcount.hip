@@ -58,7 +58,7 @@ template <> __device__ Cnt prsq_<Cnt>(Cnt x) { return opc::tally(g_ops.sqrt, 1.0
 template <> __device__ Cnt inv_<Cnt>(Cnt x) { return opc::tally(g_ops.rcp, 1.0 / x.v, x.var); }
 template <> __device__ Cnt pinv_<Cnt>(Cnt x) { return opc::tally(g_ops.rcp, 1.0 / x.v, x.var); }
 template <> __device__ Cnt div_<Cnt>(Cnt a, Cnt b) { return opc::tally(g_ops.div, a.v / b.v, a.var || b.var); }
-template <> __device__ Cnt cbrt_<Cnt>(Cnt x) { return opc::tally(g_ops.cbrt, cbrt(x.v), x.var); }
+template <> __device__ Cnt prcbrt_<Cnt>(Cnt x) { return opc::tally(g_ops.cbrt, 1.0 / cbrt(x.v), x.var); }
 }  // namespace dev
 
 namespace opc {

@@ -371,16 +371,28 @@ inline ViscParams make_visc_params(double mach, double reyl, double pr, double f
     return v;
 }
 
-template <typename T> __device__ __forceinline__ T cbrt_(T x);
-template <> __device__ __forceinline__ double cbrt_<double>(double x) { return cbrt(x); }
-template <> __device__ __forceinline__ float cbrt_<float>(float x) { return cbrtf(x); }
+// x^(-1/3) of a positive normal operand (a det): the float estimate 2^(-log2(x) / 3) of the
+// hardware log2 and exp2, refined by one Newton step y <- y + y (1 - x y^3) / 3 in float and two
+// more in double.
+template <typename T> __device__ __forceinline__ T prcbrt_(T x);
+template <> __device__ __forceinline__ float prcbrt_<float>(float x)
+{
+    const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (-1.0f / 3.0f));
+    return __builtin_fmaf(y * (1.0f / 3.0f), __builtin_fmaf(-x * y, y * y, 1.0f), y);
+}
+template <> __device__ __forceinline__ double prcbrt_<double>(double x)
+{
+    double y = (double)prcbrt_<float>((float)x);
+    y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x * y, y * y, 1.0), y);
+    return __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x * y, y * y, 1.0), y);
+}
 
 template <typename T>
 __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], T u, T v, T w, const T (&m)[9], T D,
                                           const T (&KS)[3], const ViscParams &vp, T (&fvis)[3][5])
 {
     const T C_MTS = T(vp.c_mts);
-    const T iU0 = inv_(U[0]);
+    const T iU0 = pinv_(U[0]);
     const T iU02 = iU0 * iU0;
     T ug[3], vg[3], wg[3], tg[3];
 #pragma unroll
@@ -408,8 +420,8 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
 
     // Viscous.cl:455-458: T = gamma M^2 (gamma-1) e, Sutherland mu with S = 110.4
     const T d_temp = T(vp.tfac) * (U[4] * iU0 - T(0.5) * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * iU02));
-    const T st = sqrt_(d_temp);
-    const T mu = (st * st * st) * div_(T(vp.suth1), d_temp + T(vp.suth));
+    const T st = psqrt_(d_temp);
+    const T mu = (st * st * st) * (T(vp.suth1) * pinv_(d_temp + T(vp.suth)));
     // Viscous.cl:461-476 heat flux and Stokes stresses
     const T kq = -(mu * T(vp.kheat));
     const T qx = kq * ex, qy = kq * ey, qz = kq * ez;
@@ -424,7 +436,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
     const T sxy = T(0.5) * (uy + vx), syz = T(0.5) * (vz + wy), sxz = T(0.5) * (wx + uz);
     const T sxx = ux, syy = vy, szz = wz;
     const T mod = sqrt_(T(2) * ((sxy * sxy + syz * syz + sxz * sxz) + T(2.0 / 3.0) * (sxx * sxx + syy * syy + szz * szz)));
-    const T DEL = inv_(cbrt_(D));
+    const T DEL = prcbrt_(D);
     T eddy[3];
     if (vp.sgs == 1) {
         // Smagorinsky (no reference counterpart): nu_t = (C_s Delta)^2 |S| with Delta = det^(1/3) and
@@ -440,7 +452,7 @@ __device__ __forceinline__ T viscous_node(const T (&dU)[5][3], const T (&U)[5], 
             if (mod != T(0)) {
                 // C_MTS / (1 + R / C_T) * DEL * sk with R = DEL |S| / sk, written with one division
                 // (sk = 0 gives R = inf and a zero eddy viscosity either way)
-                eddy[d] = div_(C_MTS * DEL * sk * sk, sk + DEL * mod * T(vp.inv_ct));
+                eddy[d] = (C_MTS * DEL * sk * sk) * pinv_(sk + DEL * mod * T(vp.inv_ct));
             } else {
                 eddy[d] = C_MTS * DEL * sk;
             }
