@@ -399,11 +399,11 @@ __global__ void __launch_bounds__(256, (FACE_WAVES<DIR, TA, TB>)) k_face(const B
         // direction: the first face lies between the first owned node and its ghost)
         const int nfa = faces_along(B, DIR), sh = B.off[DIR] ? 1 : 0;
         const uint32_t nta = (n_across + TA - 1) / TA, ntb = (nfa + TB - 1) / TB;
-        const uint32_t nfix = DIR == 2 ? B.nj : B.nk;
+        // tiles run along the lines' direction before the fixed coordinate, so that the next tile of
+        // an XCD re-reads the stencil halo planes of this one from its L2
         int b0, a0, fixed;
         if (DIR == 0) { b0 = (lt % ntb) * TB; a0 = ((lt / ntb) % nta) * TA; fixed = lt / (ntb * nta); }
-        else if (DIR == 1) { a0 = (lt % nta) * TA; b0 = ((lt / nta) % ntb) * TB; fixed = lt / (nta * ntb); }
-        else { a0 = (lt % nta) * TA; fixed = (lt / nta) % nfix; b0 = (lt / (nta * nfix)) * TB; }
+        else { a0 = (lt % nta) * TA; b0 = ((lt / nta) % ntb) * TB; fixed = lt / (nta * ntb); }
         const S *P0 = (const S *)B.P[0], *P1 = (const S *)B.P[1], *P2 = (const S *)B.P[2], *P3 = (const S *)B.P[3], *P4 = (const S *)B.P[4];
 
         __syncthreads();   // the previous tile's LDS fully consumed
