@@ -225,12 +225,12 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
     const T beta = irho * ia * s2;                      // 1 / (sqrt2 rho a)
     const T g4aa = T(0.4) * iaa;
     const T one_m = T(1) - phi * iaa;
-    const T cv = kz * v - ky * w, cw = kx * w - kz * u, cu = ky * u - kx * v;  // n x u components
+    const T cv = kz * v - ky * w, cw = kx * w - kz * u, cu = ky * u - kx * v;  // u x k
     const T phi_g = phi * T(2.5);
     const T hh = (phi + a * a) * T(2.5);
 
     // Projection W = L q of the 6 stencil states, written through L's structure: rows 0-2 are
-    // k_c E + (1/rho) (k x q_mom - (k x u) q0)_c with E = (1 - phi/a^2) q0 + (0.4/a^2) m and
+    // k_c E + (1/rho) (q_mom x k - (u x k) q0)_c with E = (1 - phi/a^2) q0 + (0.4/a^2) m and
     // m = u.q_mom - q4; rows 3 and 4 are beta (phi q0 - 0.4 m) -/+ beta a (theta q0 - k.q_mom).
     const T ikx = irho * kx, iky = irho * ky, ikz = irho * kz;
     const T icu = irho * cu, icv = irho * cv, icw = irho * cw;
@@ -253,9 +253,9 @@ __device__ __forceinline__ void characteristic_states(const T (&q)[6][5], const 
     T xp[5], xn[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) wcns_pair(W[c], eps, xp[c], xn[c]);
-    // Back projection Q = R x through R's structure: with X = k.(x0, x1, x2),
-    // P = alpha (x3 + x4) and M = alpha a (x3 - x4): Q0 = X + P, Q_mom = u Q0 + k M + rho (k x x),
-    // Q4 = (phi/0.4) X + H P + theta M + rho (k x u).(x0, x1, x2)
+    // Back projection Q = R x through R's structure: with x' = (x0, x1, x2), X = k.x',
+    // P = alpha (x3 + x4) and M = alpha a (x3 - x4): Q0 = X + P, Q_mom = u Q0 + k M + rho (k x x'),
+    // Q4 = (phi/0.4) X + H P + theta M + rho (u x k).x'
     const T rkx = rho * kx, rky = rho * ky, rkz = rho * kz;
     const T rcu = rho * cu, rcv = rho * cv, rcw = rho * cw;
     const T aa = alpha * a;
