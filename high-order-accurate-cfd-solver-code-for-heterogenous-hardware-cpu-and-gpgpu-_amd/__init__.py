@@ -151,6 +151,7 @@ EXPORTS = {
     "cfdx_block_get_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_batch_set_tiles": (C.c_int, [_vp, C.POINTER(Tiles)]),
     "cfdx_op_count": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(_i64)]),
+    "cfdx_math_eval": (C.c_int, [_vp, C.c_int, _vp, _vp, _i64]),
     "cfdx_block_set_monitor": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_set_check": (C.c_int, [_vp, C.c_int]),
     "cfdx_block_residual": (C.c_int, [_vp, C.POINTER(_d)]),
@@ -458,6 +459,20 @@ class Ctx:
             d["total"] = sum(d.values())
             res[name] = d
         return res
+
+    MATH_FNS = ("pinv", "psqrt", "prsq", "prcbrt")
+
+    def math_eval(self, fn: str, x, precision: str = "fp64"):
+        """The kernels' positive-operand helper `fn` (pinv 1/x, psqrt sqrt x, prsq 1/sqrt x,
+        prcbrt x^(-1/3)) evaluated on the device in the plain (fp64) or mixed (fp32) form
+        (cfdx_math_eval): a float64 array of the results."""
+        if fn not in self.MATH_FNS or precision not in ("fp64", "fp32"):
+            raise ValueError(f"math_eval: unknown helper {fn!r} / precision {precision!r}")
+        code = self.MATH_FNS.index(fn) + (4 if precision == "fp32" else 0)
+        xa = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(xa)
+        _check(_lib.cfdx_math_eval(self.h, code, xa.ctypes.data, y.ctypes.data, xa.size))
+        return y
 
     def record(self, slot: int) -> None:
         _check(_lib.cfdx_event_record(self.h, slot))

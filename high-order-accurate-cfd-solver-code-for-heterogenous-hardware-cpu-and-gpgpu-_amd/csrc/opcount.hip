@@ -162,3 +162,59 @@ extern "C" int cfdx_op_count(cfdx_ctx *ctx, const cfdx_params *params, int64_t o
     }
     return CFDX_OK;
 }
+
+namespace cfdx {
+namespace opc {
+
+// y[t] = the device helper fn at x[t]: 0..3 the double forms of the positive-operand reciprocal,
+// square root, reciprocal square root and inverse cube root, 4..7 their float forms (operand
+// rounded to float, result widened)
+__global__ void __launch_bounds__(256) k_math_eval(int fn, const double *__restrict__ x, double *__restrict__ y,
+                                                   int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double v = x[t];
+    const float f = (float)v;
+    double r = 0.0;
+    switch (fn) {
+    case 0: r = dev::pinv_<double>(v); break;
+    case 1: r = dev::psqrt_<double>(v); break;
+    case 2: r = dev::prsq_<double>(v); break;
+    case 3: r = dev::prcbrt_<double>(v); break;
+    case 4: r = (double)dev::pinv_<float>(f); break;
+    case 5: r = (double)dev::psqrt_<float>(f); break;
+    case 6: r = (double)dev::prsq_<float>(f); break;
+    default: r = (double)dev::prcbrt_<float>(f); break;
+    }
+    y[t] = r;
+}
+
+}  // namespace opc
+}  // namespace cfdx
+
+extern "C" int cfdx_math_eval(cfdx_ctx *ctx, int fn, const double *x, double *y, int64_t n)
+{
+    CFDX_TRY(cfdx::enter(ctx));
+    if (fn < 0 || fn > 7) return cfdx::fail(CFDX_E_INVALID, "math_eval: fn must be 0..7");
+    if (n < 0 || n > (int64_t(1) << 24)) return cfdx::fail(CFDX_E_INVALID, "math_eval: n must be 0..2^24");
+    if (n == 0) return CFDX_OK;
+    if (!x || !y) return cfdx::fail(CFDX_E_INVALID, "math_eval: null array");
+    double *dev = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    CFDX_HIP(hipMalloc((void **)&dev, 2 * bytes), "math_eval hipMalloc");
+    hipError_t e = hipMemcpyAsync(dev, x, bytes, hipMemcpyHostToDevice, ctx->stream);
+    int rc = e == hipSuccess ? CFDX_OK : cfdx::hip_fail(e, "math_eval copy in");
+    if (rc == CFDX_OK) {
+        hipLaunchKernelGGL(cfdx::opc::k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, fn, dev,
+                           dev + n, n);
+        rc = cfdx::check_launch("k_math_eval");
+    }
+    if (rc == CFDX_OK) {
+        e = hipMemcpyAsync(y, dev + n, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = cfdx::hip_fail(e, "math_eval copy out");
+    }
+    (void)hipFree(dev);
+    return rc;
+}

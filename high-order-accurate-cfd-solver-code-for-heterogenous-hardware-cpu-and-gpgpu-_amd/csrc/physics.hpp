@@ -27,10 +27,11 @@ template <> __device__ __forceinline__ float div_<float>(float a, float b) { ret
 
 // Reciprocal, square root and reciprocal square root of a strictly positive normal operand (a
 // density, a squared norm, a sum of positive terms): the hardware estimate (relative error below
-// 6e-8 on gfx950) refined in double by one Newton-Raphson step (reciprocal: within 10 ulp, 0.4 ulp
-// on average) or one Goldschmidt step and one residual correction (square root: correctly rounded
-// on 4M random operands in [1e-3, 1e3]); the hardware instructions in float. Not for 0, inf or
-// NaN operands (no range scaling).
+// 6e-8 on gfx950) refined in double by one Newton-Raphson step (reciprocal: relative error below
+// 2.2e-15, 0.7 ulp on average) or one Goldschmidt step and one residual correction (square root:
+// within 0.5 ulp, correctly rounded for 99.98 % of operands; its reciprocal within 1.5 ulp); the
+// hardware instructions in float (within 1 ulp). Measured on 2^20 log-uniform operands in
+// [1e-300, 1e300]. Not for 0, inf, NaN or operands whose result is subnormal (no range scaling).
 template <typename T> __device__ __forceinline__ T pinv_(T x);
 template <> __device__ __forceinline__ double pinv_<double>(double x)
 {
@@ -373,7 +374,7 @@ inline ViscParams make_visc_params(double mach, double reyl, double pr, double f
 
 // x^(-1/3) of a positive normal operand (a det): the float estimate 2^(-log2(x) / 3) of the
 // hardware log2 and exp2, refined by one Newton step y <- y + y (1 - x y^3) / 3 in float and two
-// more in double.
+// more in double (within 1 ulp on operands in [1e-36, 1e36]; float form within 1 ulp too).
 template <typename T> __device__ __forceinline__ T prcbrt_(T x);
 template <> __device__ __forceinline__ float prcbrt_<float>(float x)
 {

@@ -310,6 +310,13 @@ int cfdx_block_get_tiles(cfdx_block *b, cfdx_tiles *t);
  * reciprocal, cube root (a fused multiply-add counts as one add and one mul). The arithmetic
  * written out in the node kernels themselves is not included (see opcount.hip). */
 int cfdx_op_count(cfdx_ctx *ctx, const cfdx_params *params, int64_t out[24]);
+/* Device arithmetic helpers evaluated elementwise (no counterpart in the reference; for testing
+ * the kernels' positive-operand reciprocal, square root, reciprocal square root and inverse cube
+ * root): y[t] = f(x[t]) for host arrays of n <= 2^24 values, fn = 0..3 the double forms
+ * (1/x, sqrt x, 1/sqrt x, x^(-1/3)), 4..7 the float forms of the mixed precision (x rounded to
+ * float, the result widened). Operands must be positive normal numbers (of float for fn 3, whose
+ * first estimate is taken in float, and for 4..7). */
+int cfdx_math_eval(cfdx_ctx *ctx, int fn, const double *x, double *y, int64_t n);
 
 typedef struct cfdx_batch cfdx_batch;
 int cfdx_batch_create(cfdx_ctx *ctx, int precision, cfdx_batch **out);

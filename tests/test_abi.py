@@ -47,6 +47,7 @@ def test_error_codes_without_device():
     assert b"periodic_table" in L.cfdx_last_error()
     assert L.cfdx_uq(None, None, None, None, None, None, None, 10) == cfdx.E_INVALID
     assert L.cfdx_block_step(None, 0.5, 0.01, 1, None) == cfdx.E_INVALID
+    assert L.cfdx_math_eval(None, 0, None, None, 4) == cfdx.E_INVALID
     d = cfdx.BlockDesc(0, 4, 4, 7, 0)
     assert L.cfdx_block_bytes(C.byref(d)) == -1
     d = cfdx.BlockDesc(64, 64, 64, 7, 0)
