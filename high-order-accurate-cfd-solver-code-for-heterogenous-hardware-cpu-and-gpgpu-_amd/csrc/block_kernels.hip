@@ -769,8 +769,10 @@ __global__ void __launch_bounds__(256) k_bc(const BlockDev *__restrict__ blks, i
     const uint32_t nf = (uint32_t)na * (uint32_t)nb;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 4 * nf) return;
-    const int m = (int)(t / nf);
-    const uint32_t r = t - (uint32_t)m * nf;
+    // i sides: the four layers of a face node are adjacent in memory, so they go to four adjacent
+    // threads; j and k sides: adjacent threads take adjacent i of one layer
+    const int m = d == 0 ? (int)(t & 3u) : (int)(t / nf);
+    const uint32_t r = d == 0 ? t >> 2 : t - (uint32_t)m * nf;
     const int a = (int)(r % (uint32_t)na), bq = (int)(r / (uint32_t)na);
     const int n = d == 0 ? B.ni : d == 1 ? B.nj : B.nk;
     auto at = [&](int c) -> uint32_t {
