@@ -69,6 +69,14 @@ namespace {
 int wait_batches(cfdx_block *b)
 {
     if (!b->left_pending) return CFDX_OK;
+    // launches that have completed need no stream dependency (and a dependency on the batch's
+    // stream while another thread captures it is refused)
+    const hipError_t q = hipEventQuery(b->left);
+    if (q == hipSuccess) {
+        b->left_pending = false;
+        return CFDX_OK;
+    }
+    if (q != hipErrorNotReady) return cfdx::hip_fail(q, "query the batch the block left");
     CFDX_HIP(hipStreamWaitEvent(b->ctx->stream, b->left, 0), "wait for the batch the block left");
     return CFDX_OK;
 }

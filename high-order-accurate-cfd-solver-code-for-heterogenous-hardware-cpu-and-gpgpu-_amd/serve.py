@@ -135,10 +135,12 @@ def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) ->
 
 
 class Server:
-    """Continuous batching on one GPU. The batched steps run on the main context's stream; blocks
-    are created, set up and read back on a second context (its own stream) while a chunk of
-    steps runs, so block set-up, result copies and host work overlap the device work (pipelined:
-    one chunk in flight). Memory of retired blocks is released after the next chunk completes."""
+    """Continuous batching on one GPU. The batched steps run on the main context's stream. With
+    the wall clock, blocks are created, set up (each on a stream of its own), read back and
+    released by a set-up thread while chunks of steps run, so block set-up, result copies and host
+    work overlap the device work (pipelined: one chunk in flight; the chunk loop itself never waits
+    for an upload); with the steps clock the same happens on the calling thread and a second
+    context. Memory of retired blocks is released after the next chunk completes."""
 
     def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
                  budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
@@ -151,7 +153,7 @@ class Server:
         # ones keep arriving)
         self.policy = policy
         self.ctx = ctx or Ctx()
-        # block set-up and result copies (same device, another stream)
+        # block set-up and result copies on the calling thread (same device, another stream)
         self.io = Ctx(self.ctx.device) if pipelined else self.ctx
         self.pipelined = pipelined
         self.precision = precision
@@ -171,15 +173,21 @@ class Server:
         self.peak_bytes = 0
         self.chunks = 0
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, prep_workers))
+        # one thread for every call on the set-up context (its stream is used in call order)
+        self.setup = concurrent.futures.ThreadPoolExecutor(max_workers=1)
 
-    def _fits(self, req: Request, used: int, nodes: int = 0, busy: bool = False) -> bool:
-        if self.max_nodes is not None and busy and nodes + req.nodes > self.max_nodes:
+    def _fits(self, req: Request, used: int, nodes: int = 0, busy: bool = False, cap: int = 1,
+              unallocated: int = 0) -> bool:
+        """Whether `req` may be admitted: nodes (in flight and set up) stay within cap x max_nodes,
+        the bytes within the budget, and the device keeps the reserve free after `unallocated`
+        bytes of blocks still being set up."""
+        if self.max_nodes is not None and busy and nodes + req.nodes > cap * self.max_nodes:
             return False
         need = req.bytes(self.precision)
         if self.budget is not None and used + need > self.budget:
             return False
         free, _ = self.ctx.mem_info()
-        return free - need >= self.reserve
+        return free - unallocated - need >= self.reserve
 
     def run(self, requests: list[Request], clock: str = "wall") -> dict:
         if clock not in ("wall", "steps"):
@@ -188,23 +196,62 @@ class Server:
         queue: list[Request] = []          # arrived, not admitted; admission takes queue[0]
         order = (lambda r: (r.nodes * r.nsteps, r.arrival, r.rid)) if self.policy == "sjf" else None
         active: dict[int, dict] = {}      # in the batch
-        admitted: list[dict] = []          # set up, joining the batch at the next chunk boundary
+        setting_up: list[dict] = []        # admitted, block being set up on the set-up thread
+        admitted: list[dict] = []          # set up, joining the batch at a chunk boundary
+        finishing: list[dict] = []         # done, result being read back on the set-up thread
         results: dict[int, Result] = {}
         retired: list = []                 # blocks whose memory is released after the next sync
+        closing: list = []                 # releases running on the set-up thread
         used = 0
         vclock = 0                         # steps executed (steps clock)
         inflight = 0                       # steps of the chunk running on the device (0: none)
         t0 = time.perf_counter()
+        # wall clock: block set-up, result read-back and release run on the set-up thread beside
+        # the batched chunks, so the chunk loop never waits for an upload; blocks are set up ahead,
+        # up to max_nodes more nodes than are in flight. Steps clock: on the calling thread and the
+        # set-up context, so that a schedule is reproducible.
+        threaded = clock == "wall" and self.pipelined
 
         def now() -> float:
             return float(vclock) if clock == "steps" else time.perf_counter() - t0
 
         def nodes_in_flight() -> int:
-            return sum(a["req"].nodes for a in active.values()) + sum(a["req"].nodes for a in admitted)
+            return (sum(a["req"].nodes for a in active.values()) + sum(a["req"].nodes for a in admitted)
+                    + sum(a["req"].nodes for a in setting_up))
+
+        def set_up(req: Request, fut):
+            # threaded: each block on a stream of its own, so that the batch it joins waits for
+            # its set-up only, not for the uploads of the blocks set up after it
+            c = Ctx(self.ctx.device) if threaded else self.io
+            try:
+                return make_block(req, self.precision, c, fut.result() if fut is not None else None)
+            finally:
+                if threaded:
+                    c.close()   # the block holds the context until it is destroyed
+
+        def read_back(b: Block):
+            # a diverged solve is reported as such (Result.finite, report()["nonfinite"]), not
+            # returned as a NaN field
+            finite = b.check_finite()
+            flow = b.get_flow() if self.keep_flow else None
+            return finite, flow, time.perf_counter() - t0
+
+        def finish(a: dict, finite: bool, flow, t_done: float) -> None:
+            rq = a["req"]
+            results[rq.rid] = Result(rq.rid, rq.arrival, a["admitted"], t_done, a["dt"], flow, finite)
+            retired.append(a["block"])
+
+        def release(blocks: list) -> None:
+            for b in blocks:
+                if threaded:
+                    closing.append(self.setup.submit(b.close))
+                else:
+                    b.close()
+            blocks.clear()
 
         preps: dict[int, concurrent.futures.Future] = {}
         try:
-            while pending or queue or active or admitted or inflight:
+            while pending or queue or active or admitted or setting_up or finishing or inflight:
                 t = now()
                 while pending and pending[0].arrival <= t:
                     req = pending.popleft()
@@ -214,24 +261,43 @@ class Server:
                         bisect.insort(queue, req, key=order)
                     if req.grid == "cgrid":
                         preps[req.rid] = self.pool.submit(prepare, req)
-                # admission in queue order while memory, slots and the node cap allow, once the host set-up is
-                # ready (waited for when nothing else could run); the block is built on the set-up
-                # stream while a chunk may be running
-                busy = bool(active or admitted)
-                while queue and len(active) + len(admitted) < self.max_active and self._fits(
-                        queue[0], used, nodes_in_flight(), busy):
+                # set-ups and read-backs that completed, in order
+                while setting_up and setting_up[0]["fut"].done():
+                    a = setting_up.pop(0)
+                    a["block"], a["dt"] = a.pop("fut").result()
+                    admitted.append(a)
+                while finishing and finishing[0]["fut"].done():
+                    a = finishing.pop(0)
+                    finish(a, *a.pop("fut").result())
+                # admission in queue order while memory, slots and the node cap allow; on the
+                # calling thread once the host set-up is ready (waited for when nothing else
+                # could run)
+                busy = bool(active or admitted or setting_up)
+                while queue and len(active) + len(admitted) + len(setting_up) < self.max_active and self._fits(
+                        queue[0], used, nodes_in_flight(), busy, cap=2 if threaded else 1,
+                        unallocated=sum(a["bytes"] for a in setting_up)):
                     req = queue[0]
                     fut = preps.get(req.rid)
-                    if fut is not None and not fut.done() and (busy or inflight):
+                    if not threaded and fut is not None and not fut.done() and (busy or inflight):
                         break
                     queue.pop(0)
-                    prep = preps.pop(req.rid).result() if fut is not None else None
-                    b, dt = make_block(req, self.precision, self.io, prep)
+                    fut = preps.pop(req.rid, None)
                     nb = req.bytes(self.precision)
                     used += nb
-                    admitted.append(dict(req=req, block=b, left=req.nsteps, admitted=now(), dt=dt, bytes=nb))
+                    a = dict(req=req, left=req.nsteps, admitted=now(), bytes=nb)
+                    if threaded:
+                        a["fut"] = self.setup.submit(set_up, req, fut)
+                        setting_up.append(a)
+                    else:
+                        a["block"], a["dt"] = set_up(req, fut)
+                        admitted.append(a)
                     busy = True
-                if queue and not (active or admitted or inflight):
+                if queue and not (active or admitted or setting_up or finishing or inflight):
+                    if retired or not all(f.done() for f in closing):
+                        # the blocks released last still hold their memory: release them, then retry
+                        release(retired)
+                        concurrent.futures.wait(closing)
+                        continue
                     raise RuntimeError(f"request {queue[0].rid} ({queue[0].ni}x{queue[0].nj}x{queue[0].nk}) does not fit "
                                        "in the free device memory")
                 # the chunk in flight completes
@@ -245,16 +311,17 @@ class Server:
                         if a["left"] == 0:
                             done.append(active[rid])
                     inflight = 0
-                    for b in retired:
-                        b.close()
-                    retired = []
+                    release(retired)
                 for a in done:
                     active.pop(a["req"].rid)
                     self.batch.remove(a["block"])
-                for a in admitted:
+                # set-up blocks join in admission order while the in-flight node cap allows (the
+                # first always joins an empty batch)
+                while admitted and (self.max_nodes is None or not active or sum(
+                        x["req"].nodes for x in active.values()) + admitted[0]["req"].nodes <= self.max_nodes):
+                    a = admitted.pop(0)
                     self.batch.add(a["block"])
                     active[a["req"].rid] = a
-                admitted = []
                 # next chunk, asynchronous
                 if active:
                     self.peak_active = max(self.peak_active, len(active))
@@ -266,35 +333,39 @@ class Server:
                     inflight = k
                     if not self.pipelined:
                         self.ctx.sync()
-                # results of the requests that finished, copied on the set-up stream during the chunk
+                # results of the requests that finished, read back on the set-up stream during the chunk
                 for a in done:
-                    b = a["block"]
-                    # a diverged solve is reported as such (Result.finite, report()["nonfinite"]),
-                    # not returned as a NaN field
-                    finite = b.check_finite()
-                    flow = b.get_flow() if self.keep_flow else None
                     used -= a["bytes"]
-                    results[a["req"].rid] = Result(a["req"].rid, a["req"].arrival, a["admitted"], now(), a["dt"], flow,
-                                                   finite)
-                    retired.append(b)
-                if not (active or admitted or inflight) and pending and not queue:
-                    for b in retired:
-                        b.close()
-                    retired = []
-                    if clock == "steps":   # idle until the next arrival
-                        vclock = max(vclock, int(math.ceil(pending[0].arrival)))
+                    if threaded:
+                        a["fut"] = self.setup.submit(read_back, a["block"])
+                        finishing.append(a)
                     else:
-                        time.sleep(max(0.0, pending[0].arrival - now()))
-            for b in retired:
-                b.close()
+                        finite, flow, _ = read_back(a["block"])
+                        finish(a, finite, flow, now())
+                if not (active or inflight):
+                    if setting_up and not admitted:
+                        concurrent.futures.wait([setting_up[0]["fut"]])   # nothing to step until it is set up
+                    elif finishing and not (admitted or setting_up):
+                        concurrent.futures.wait([finishing[0]["fut"]])
+                    elif pending and not (queue or admitted or setting_up or finishing):
+                        release(retired)
+                        if clock == "steps":   # idle until the next arrival
+                            vclock = max(vclock, int(math.ceil(pending[0].arrival)))
+                        else:
+                            time.sleep(max(0.0, pending[0].arrival - now()))
+            release(retired)
+            concurrent.futures.wait(closing)
+            for f in closing:
+                f.result()
             return results
         except BaseException:
-            self._abandon(active, admitted, retired)
+            self._abandon(active, admitted, retired, setting_up, finishing, closing)
             raise
 
-    def _abandon(self, active: dict, admitted: list, retired: list) -> None:
-        """After a failure inside run(): waits for the device and releases every block the run
-        holds, so that the server and its batch can be used again."""
+    def _abandon(self, active: dict, admitted: list, retired: list, setting_up: list = (), finishing: list = (),
+                 closing: list = ()) -> None:
+        """After a failure inside run(): waits for the device and the set-up thread and releases
+        every block the run holds, so that the server and its batch can be used again."""
         try:
             self.ctx.sync()
         except Exception:
@@ -304,7 +375,20 @@ class Server:
                 self.batch.remove(a["block"])
             except Exception:
                 pass
-        for b in [a["block"] for a in active.values()] + [a["block"] for a in admitted] + list(retired):
+        blocks = [a["block"] for a in active.values()] + [a["block"] for a in admitted] + list(retired)
+        for a in setting_up:
+            try:
+                blocks.append(a["fut"].result()[0])
+            except Exception:
+                pass
+        for a in finishing:
+            try:
+                a["fut"].result()
+            except Exception:
+                pass
+            blocks.append(a["block"])
+        concurrent.futures.wait(list(closing))
+        for b in blocks:
             try:
                 b.close()
             except Exception:
@@ -313,6 +397,7 @@ class Server:
     def close(self) -> None:
         self.batch.close()
         self.pool.shutdown(wait=True)
+        self.setup.shutdown(wait=True)
         if self.io is not self.ctx:
             self.io.close()
 

@@ -153,6 +153,32 @@ def test_serving_wall_clock(ctx):
     print("serving (wall clock):", rep)
     assert rep["requests"] == len(reqs) and rep["cell_updates_per_s"] > 0
     assert all(res[r.rid].finished >= res[r.rid].admitted >= r.arrival - 1e-9 for r in reqs)
+    # wall clock: blocks set up, read back and released on the set-up thread beside the chunks
+    for r in reqs:
+        alone = serve.run_alone(r, cfdx.MIXED, ctx)
+        for f in FIELDS:
+            assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
+
+
+def test_serving_wall_clock_set_up_ahead(ctx):
+    """Wall clock with a node cap of about two requests: blocks are set up ahead on the set-up
+    thread (up to twice the cap) and join the batch when the cap allows; every result is bitwise
+    the request alone and the batch never holds more than the cap (or one request)."""
+    reqs = small_stream()
+    for r in reqs:
+        r.arrival = 0.0
+    cap = 2 * max(r.nodes for r in reqs)
+    srv = serve.Server(precision=cfdx.FP64, ctx=ctx, reserve_bytes=0, chunk=2, max_nodes=cap)
+    res = srv.run(reqs, clock="wall")
+    srv.close()
+    assert sorted(res) == [r.rid for r in reqs]
+    assert 2 <= srv.peak_active < len(reqs)
+    for r in reqs:
+        assert res[r.rid].finished >= res[r.rid].admitted >= 0.0
+        alone = serve.run_alone(r, cfdx.FP64, ctx)
+        assert res[r.rid].dt == alone.dt
+        for f in FIELDS:
+            assert np.array_equal(res[r.rid].flow[f], alone.flow[f]), (r.rid, f)
 
 
 def test_serving_airfoils_alone_vs_mix(ctx):
