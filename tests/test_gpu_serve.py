@@ -122,19 +122,24 @@ def test_serving_sjf_admits_least_work_first(ctx):
         serve.Server(precision=cfdx.FP64, ctx=ctx, policy="lifo")
 
 
-def test_serving_failure_releases_blocks(ctx):
+@pytest.mark.parametrize("clock", ["steps", "wall"])
+def test_serving_failure_releases_blocks(ctx, clock):
     """A request that fails in its set-up while another is on the GPU: run() raises, every block
     of the run is released and the batch is empty, and the same server then serves a valid
-    stream with the results of each request alone."""
+    stream with the results of each request alone (wall clock: the set-up fails on the set-up
+    thread)."""
     reqs = small_stream()[:2]
-    reqs[1].arrival = 2
+    reqs[1].arrival = 2 if clock == "steps" else 0.02
     reqs[1].init = "no-such-state"
     srv = serve.Server(precision=cfdx.FP64, ctx=ctx, reserve_bytes=0, chunk=2)
     with pytest.raises(Exception):
-        srv.run(reqs, clock="steps")
+        srv.run(reqs, clock=clock)
     assert len(srv.batch) == 0
     good = small_stream()[2:4]
-    res = srv.run(good, clock="steps")
+    if clock == "wall":
+        for r in good:
+            r.arrival = 0.0
+    res = srv.run(good, clock=clock)
     srv.close()
     for r in good:
         alone = serve.run_alone(r, cfdx.FP64, ctx)
