@@ -23,7 +23,10 @@ constexpr int NODE_TILES[N_NODE][2] = {{64, 4}, {32, 8}, {128, 2}, {256, 1}};
 struct TileCfg {
     int face[3] = {0, 0, 0};   // variant of the i, j, k face sweeps
     int node = 0;              // variant of the node kernels
-    int strip = 0;             // node tile order: strips of this many tile rows (0: whole planes)
+    // node tile order: strips of this many tile rows (0: whole planes). Strips of 4 rows by
+    // default: a thin block's planes, split over the 8 XCDs, would otherwise give each XCD only a
+    // few planes, whose k +- 2 neighbours another XCD reads (C-grid 384x128x32: +1-2 %)
+    int strip = 4;
     bool operator==(const TileCfg &o) const
     {
         return face[0] == o.face[0] && face[1] == o.face[1] && face[2] == o.face[2] && node == o.node &&

@@ -288,7 +288,8 @@ int cfdx_block_timing_split(cfdx_block *b, double ms[5]);
  * (shape = lines across j, faces along i), kind 1 = face sweeps along j and k (lines across i,
  * faces along j or k), kind 2 = node kernels (nodes along i, along j). cfdx_tile_variants gives
  * the count and, if shapes is not NULL, 2 * count shape numbers. A block or batch launches the
- * variants its cfdx_tiles names (all 0 by default); results do not depend on them. */
+ * variants its cfdx_tiles names (variants 0 and strips of 4 rows by default); results do not
+ * depend on them. */
 typedef struct {
     int32_t face[3];   /* variant of the i, j, k face sweeps */
     int32_t node;      /* variant of the node kernels */

@@ -273,5 +273,5 @@ def test_strip_out_of_range_refused(ctx):
     assert e.value.code == cfdx.E_INVALID and "strip" in str(e.value)
     with pytest.raises(cfdx.CfdxError):
         b.set_tiles((0, 0, 0, 0, -1))
-    assert b.get_tiles() == (0, 0, 0, 0, 0)
+    assert b.get_tiles() == (0, 0, 0, 0, 4)   # the default: strips of 4 tile rows
     b.close()
