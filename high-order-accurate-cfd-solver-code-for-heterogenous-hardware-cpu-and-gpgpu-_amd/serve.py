@@ -268,7 +268,12 @@ class Server:
                     admitted.append(a)
                 while finishing and finishing[0]["fut"].done():
                     a = finishing.pop(0)
-                    finish(a, *a.pop("fut").result())
+                    try:
+                        out = a.pop("fut").result()
+                    except BaseException:
+                        retired.append(a["block"])   # released by the failure path
+                        raise
+                    finish(a, *out)
                 # admission in queue order while memory, slots and the node cap allow; on the
                 # calling thread once the host set-up is ready (waited for when nothing else
                 # could run)
