@@ -22,8 +22,10 @@ from __future__ import annotations
 import bisect
 import collections
 import concurrent.futures
+import contextlib
 import dataclasses
 import math
+import threading
 import time
 
 import numpy as np
@@ -97,29 +99,42 @@ def prepare(req: Request) -> dict | None:
     return {"desc": d, "geo": geo, "flow": freestream(req.nodes, req.mach, req.aoa_deg)}
 
 
-def make_block(req: Request, precision: int, ctx: Ctx, prep: dict | None = None) -> tuple[Block, float]:
+def make_block(req: Request, precision: int, ctx: Ctx, prep: dict | None = None,
+               lock=None) -> tuple[Block, float]:
     """Creates, initialises and configures the block of one request; returns it with its CFL time
-    step (computed once from the initial state)."""
+    step (computed once from the initial state). `lock`, if given, is held around each library
+    call (see Server)."""
+    g = lock if lock is not None else contextlib.nullcontext()
     if req.grid == "cgrid":
         prep = prep or prepare(req)
         d = BlockDesc.from_buffer_copy(prep["desc"])
-        b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx, desc=d)
+        with g:
+            b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx, desc=d)
     else:
-        b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx)
+        with g:
+            b = Block(req.ni, req.nj, req.nk, precision=precision, ctx=ctx)
     try:
         if req.grid == "cgrid":
-            b.set_geometry_soa(prep["geo"])
-            b.configure(req.mach, 0.0, req.params())      # the side conditions use the block's Mach
-            b.set_flow(prep["flow"])
+            with g:
+                b.set_geometry_soa(prep["geo"])
+            with g:
+                b.configure(req.mach, 0.0, req.params())      # the side conditions use the block's Mach
+            with g:
+                b.set_flow(prep["flow"])
         else:
             amp = req.amp_frac * req.length[0] / req.ni if req.grid == "sinusoid" else 0.0
-            b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init, mach=req.mach,
-                        beta=req.beta, center=req.center)
-            b.configure(req.mach, 0.0, req.params())      # the stable step uses the request's Re
-        dt = b.cfl_dt(req.cfl)
-        b.configure(req.mach, dt, req.params())
+            with g:
+                b.init_case(grid=req.grid, length=req.length, amp=amp, phases=req.phases, init=req.init,
+                            mach=req.mach, beta=req.beta, center=req.center)
+            with g:
+                b.configure(req.mach, 0.0, req.params())      # the stable step uses the request's Re
+        with g:
+            dt = b.cfl_dt(req.cfl)
+        with g:
+            b.configure(req.mach, dt, req.params())
     except BaseException:
-        b.close()
+        with g:
+            b.close()
         raise
     return b, dt
 
@@ -173,8 +188,13 @@ class Server:
         self.peak_bytes = 0
         self.chunks = 0
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, prep_workers))
-        # one thread for every call on the set-up context (its stream is used in call order)
+        # one thread for the set-up, read-back and release of blocks
         self.setup = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        # held by the set-up thread around each library call and by the chunk loop while the batch
+        # captures its step graph: the HIP runtime invalidates a capture when another thread
+        # allocates or creates streams during it (a capture failed that way with 12 M nodes in
+        # flight), so no other library call runs while a capture is open
+        self.hip_lock = threading.Lock()
 
     def _fits(self, req: Request, used: int, nodes: int = 0, busy: bool = False, cap: int = 1,
               unallocated: int = 0) -> bool:
@@ -219,22 +239,34 @@ class Server:
             return (sum(a["req"].nodes for a in active.values()) + sum(a["req"].nodes for a in admitted)
                     + sum(a["req"].nodes for a in setting_up))
 
+        lock = self.hip_lock if threaded else contextlib.nullcontext()
+
         def set_up(req: Request, fut):
             # threaded: each block on a stream of its own, so that the batch it joins waits for
             # its set-up only, not for the uploads of the blocks set up after it
-            c = Ctx(self.ctx.device) if threaded else self.io
+            prep = fut.result() if fut is not None else None
+            if not threaded:
+                return make_block(req, self.precision, self.io, prep)
+            with lock:
+                c = Ctx(self.ctx.device)
             try:
-                return make_block(req, self.precision, c, fut.result() if fut is not None else None)
+                return make_block(req, self.precision, c, prep, lock=lock)
             finally:
-                if threaded:
+                with lock:
                     c.close()   # the block holds the context until it is destroyed
 
         def read_back(b: Block):
             # a diverged solve is reported as such (Result.finite, report()["nonfinite"]), not
             # returned as a NaN field
-            finite = b.check_finite()
-            flow = b.get_flow() if self.keep_flow else None
+            with lock:
+                finite = b.check_finite()
+            with lock:
+                flow = b.get_flow() if self.keep_flow else None
             return finite, flow, time.perf_counter() - t0
+
+        def close(b: Block) -> None:
+            with lock:
+                b.close()
 
         def finish(a: dict, finite: bool, flow, t_done: float) -> None:
             rq = a["req"]
@@ -244,7 +276,7 @@ class Server:
         def release(blocks: list) -> None:
             for b in blocks:
                 if threaded:
-                    closing.append(self.setup.submit(b.close))
+                    closing.append(self.setup.submit(close, b))
                 else:
                     b.close()
             blocks.clear()
@@ -317,6 +349,7 @@ class Server:
                             done.append(active[rid])
                     inflight = 0
                     release(retired)
+                changed = bool(done)               # the batch captures its step again
                 for a in done:
                     active.pop(a["req"].rid)
                     self.batch.remove(a["block"])
@@ -327,6 +360,7 @@ class Server:
                     a = admitted.pop(0)
                     self.batch.add(a["block"])
                     active[a["req"].rid] = a
+                    changed = True
                 # next chunk, asynchronous
                 if active:
                     self.peak_active = max(self.peak_active, len(active))
@@ -334,7 +368,8 @@ class Server:
                     k = min(self.chunk, min(a["left"] for a in active.values()))
                     if clock == "steps" and pending:
                         k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
-                    self.batch.step(k, self.graph)
+                    with lock if changed else contextlib.nullcontext():
+                        self.batch.step(k, self.graph)
                     inflight = k
                     if not self.pipelined:
                         self.ctx.sync()
