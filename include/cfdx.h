@@ -331,7 +331,9 @@ int cfdx_batch_size(cfdx_batch *q);
  * that loop over the tiles. */
 int cfdx_batch_set_grid(cfdx_batch *q, uint32_t grid_cap);
 /* nsteps RK3 steps of every block; use_graph = 1 captures one step in a hipGraph on first use
- * (re-captured after the batch changes) and replays it. Asynchronous on the context stream. */
+ * (re-captured after the batch changes) and replays it. Asynchronous on the context stream. No
+ * other thread may call the library during a call that captures: the HIP runtime invalidates the
+ * capture when another thread allocates device memory or creates streams meanwhile. */
 int cfdx_batch_step(cfdx_batch *q, int nsteps, int use_graph);
 /* Tile variants of the batch's launches (the blocks' own settings apply to single-block steps). */
 int cfdx_batch_set_tiles(cfdx_batch *q, const cfdx_tiles *t);
