@@ -153,9 +153,10 @@ class Server:
     """Continuous batching on one GPU. The batched steps run on the main context's stream. With
     the wall clock, blocks are created, set up (each on a stream of its own), read back and
     released by a set-up thread while chunks of steps run, so block set-up, result copies and host
-    work overlap the device work (pipelined: one chunk in flight; the chunk loop itself never waits
-    for an upload); with the steps clock the same happens on the calling thread and a second
-    context. Memory of retired blocks is released after the next chunk completes."""
+    work overlap the device work (pipelined: one chunk in flight; before a step that captures its
+    graph the chunk loop waits at most for the one library call the set-up thread is making); with
+    the steps clock the same happens on the calling thread and a second context. Memory of retired
+    blocks is released after the next chunk completes."""
 
     def __init__(self, precision: int = MIXED, ctx: Ctx | None = None, reserve_bytes: int = 2 << 30,
                  budget_bytes: int | None = None, max_active: int = 256, chunk: int = 10, graph: bool = True,
@@ -227,7 +228,7 @@ class Server:
         inflight = 0                       # steps of the chunk running on the device (0: none)
         t0 = time.perf_counter()
         # wall clock: block set-up, result read-back and release run on the set-up thread beside
-        # the batched chunks, so the chunk loop never waits for an upload; blocks are set up ahead,
+        # the batched chunks, so the chunk loop does not wait for the uploads; blocks are set up ahead,
         # up to max_nodes more nodes than are in flight. Steps clock: on the calling thread and the
         # set-up context, so that a schedule is reproducible.
         threaded = clock == "wall" and self.pipelined
