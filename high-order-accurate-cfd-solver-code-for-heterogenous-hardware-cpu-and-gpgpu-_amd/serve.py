@@ -307,9 +307,9 @@ class Server:
                         retired.append(a["block"])   # released by the failure path
                         raise
                     finish(a, *out)
-                # admission in queue order while memory, slots and the node cap allow; on the
-                # calling thread once the host set-up is ready (waited for when nothing else
-                # could run)
+                # admission in queue order while memory, slots and the node cap allow; the block is
+                # set up on the set-up thread (wall clock), or here once its host set-up is ready
+                # (steps clock; waited for when nothing else could run)
                 busy = bool(active or admitted or setting_up)
                 while queue and len(active) + len(admitted) + len(setting_up) < self.max_active and self._fits(
                         queue[0], used, nodes_in_flight(), busy, cap=2 if threaded else 1,
