@@ -149,6 +149,50 @@ def run_alone(req: Request, precision: int, ctx: Ctx, keep_flow: bool = True) ->
     return Result(req.rid, 0.0, 0.0, 0.0, dt, flow, finite)
 
 
+class _Gate:
+    """Orders the set-up thread's library calls (`call`) and the chunk loop's graph captures
+    (`capture`): one at a time, and a waiting capture goes before the next call, so that a capture
+    waits for one call at most (a plain lock, released and taken again at once by the set-up thread
+    between its calls, let a capture wait for a whole set-up)."""
+
+    def __init__(self):
+        self._cv = threading.Condition()
+        self._busy = False
+        self._captures = 0
+        self.call = _GateSide(self, False)
+        self.capture = _GateSide(self, True)
+
+    def _enter(self, capture: bool) -> None:
+        with self._cv:
+            if capture:
+                self._captures += 1
+                while self._busy:
+                    self._cv.wait()
+                self._captures -= 1
+            else:
+                while self._busy or self._captures:
+                    self._cv.wait()
+            self._busy = True
+
+    def _exit(self) -> None:
+        with self._cv:
+            self._busy = False
+            self._cv.notify_all()
+
+
+class _GateSide:
+    def __init__(self, gate: _Gate, capture: bool):
+        self.gate, self.capture = gate, capture
+
+    def __enter__(self):
+        self.gate._enter(self.capture)
+        return self
+
+    def __exit__(self, *exc) -> bool:
+        self.gate._exit()
+        return False
+
+
 class Server:
     """Continuous batching on one GPU. The batched steps run on the main context's stream. With
     the wall clock, blocks are created, set up (each on a stream of its own), read back and
@@ -191,11 +235,11 @@ class Server:
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, prep_workers))
         # one thread for the set-up, read-back and release of blocks
         self.setup = concurrent.futures.ThreadPoolExecutor(max_workers=1)
-        # held by the set-up thread around each library call and by the chunk loop while the batch
-        # captures its step graph: the HIP runtime invalidates a capture when another thread
+        # taken by the set-up thread around each library call and by the chunk loop while the
+        # batch captures its step graph: the HIP runtime invalidates a capture when another thread
         # allocates or creates streams during it (a capture failed that way with 12 M nodes in
         # flight), so no other library call runs while a capture is open
-        self.hip_lock = threading.Lock()
+        self.gate = _Gate()
 
     def _fits(self, req: Request, used: int, nodes: int = 0, busy: bool = False, cap: int = 1,
               unallocated: int = 0) -> bool:
@@ -240,7 +284,7 @@ class Server:
             return (sum(a["req"].nodes for a in active.values()) + sum(a["req"].nodes for a in admitted)
                     + sum(a["req"].nodes for a in setting_up))
 
-        lock = self.hip_lock if threaded else contextlib.nullcontext()
+        lock = self.gate.call if threaded else contextlib.nullcontext()
 
         def set_up(req: Request, fut):
             # threaded: each block on a stream of its own, so that the batch it joins waits for
@@ -369,7 +413,7 @@ class Server:
                     k = min(self.chunk, min(a["left"] for a in active.values()))
                     if clock == "steps" and pending:
                         k = max(1, min(k, int(math.ceil(pending[0].arrival - vclock))))
-                    with lock if changed else contextlib.nullcontext():
+                    with self.gate.capture if threaded and changed else contextlib.nullcontext():
                         self.batch.step(k, self.graph)
                     inflight = k
                     if not self.pipelined:
