@@ -36,3 +36,34 @@ def test_request_params():
     e = serve.Request(0, 8, 8, 8, 1, viscous=False).params()
     assert math.isinf(e.reyl) and e.viscous == 0
     assert cfdx.Block.bytes_for(8, 8, 8, cfdx.MIXED) < cfdx.Block.bytes_for(8, 8, 8, cfdx.FP64)
+
+
+def test_gate_lets_a_waiting_capture_go_before_the_next_call():
+    """The server's gate: set-up calls and captures run one at a time, and a capture that waits
+    goes before the set-up thread's next call, so it waits for one call at most (a plain lock,
+    released and taken again at once between calls, can hold it back for all of them)."""
+    import threading
+    import time
+
+    gate = serve._Gate()
+    order = []
+    started = threading.Event()
+
+    def calls():
+        for i in range(20):
+            with gate.call:
+                order.append(("call", i))
+                started.set()
+                time.sleep(0.005)
+
+    t = threading.Thread(target=calls)
+    t.start()
+    started.wait()
+    time.sleep(0.012)
+    with gate.capture:
+        order.append(("capture", None))
+    t.join()
+    k = order.index(("capture", None))
+    # the capture came in while calls were running and went before most of them
+    assert 1 <= k <= 10, order
+    assert [x for x in order if x[0] == "call"] == [("call", i) for i in range(20)]
